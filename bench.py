@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Throughput of the ccp Brownian-dynamics step on one MI355X.
+
+Workload (the large-nucleus configuration): 8 chains of 16384 beads
+(N = 131072) in a sphere at chromatin volume fraction 0.3 (R_n = 44.387 by
+ccp's formula), WCA repulsion, heterochromatin LJ attraction eps = 0.6 kT at
+2.5 sigma (30 % het in Geometric(20) blocks), 15 % LAD beads clustered at
+chain ends with a lamina well, harmonic bonds, bending k_b = 2 kT, dt = 1e-3,
+2.5 sigma cells, FP32, contact sampling every 1000 steps into a 1024 x 1024
+uint32 map (128 beads per bin), seed 3. One step = one stochastic-Heun
+step of all beads (two force evaluations, two cell-list builds).
+
+Prints one JSON line. `value` is bead-steps/s with the state resident in HBM;
+`roofline` is the dominant kernel's algorithmic bytes per launch over its
+HIP-event-measured average duration against 8 TB/s; `cpu_baseline` is the
+FP64 scalar oracle timed on this host's cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N_CHAINS, CHAIN_LEN = 8, 16384
+N = N_CHAINS * CHAIN_LEN
+RCO = float(np.float32(1.154701))
+CVF = 0.3
+R_N = RCO / 2 + RCO / 2 * (N / CVF) ** (1.0 / 3.0)
+
+PARAMS = {
+    "number_of_particles": N,
+    "nucleus_radius": f"{R_N:.6f}",
+    "number_of_chains": N_CHAINS,
+    "force_field": "lj",
+    "epsilon_het": 0.6,
+    "epsilon_lamina": 1.0,
+    "het_fraction": 0.3,
+    "het_block_mean": 20,
+    "lad_fraction": 0.15,
+    "lad_placement": "ends",
+    "timestep": 0.001,
+    "cell_size": 2.5,
+    "contact_cutoff": 1.5,
+    "contact_bin_size": 128,
+    "contact_interval": 1000,
+    "seed": 3,
+    "graph_steps": 100,
+}
+
+# Algorithmic HBM bytes per bead for one launch of each step kernel
+# (16 B float4 positions, 4 B indices/counters; neighbour gathers counted
+# once because the sorted copy is read once per launch):
+#   force stage 1: R sorted pos 16 + R pos by index 16 (bonds) + W er 16
+#                  + W r+f dt/2+noise 16 + W bucket slot 4 + count atomic 4 = 72
+#   force stage 2: R sorted pos 16 + R er by index 16 + R r+f dt/2+noise 16
+#                  + W r 16 + W bucket slot 4 + count atomic 4 = 72
+#   order:         R bucket 4 + R pos 16 + W sorted pos 16 = 36 per bead,
+#                  + R count 4 + W start 4 per cell
+BYTES_FORCE_PER_BEAD = 72
+BYTES_ORDER_PER_BEAD = 36
+BYTES_ORDER_PER_CELL = 8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_force_lj_n131072_traffic.txt"))
+    return ap.parse_args()
+
+
+def read_traffic(path):
+    """HBM bytes per launch of the dominant kernel from a committed rocprofv3
+    PMC summary (line 'bytes_per_launch <value>'), or None."""
+    try:
+        for line in open(path):
+            if line.startswith("bytes_per_launch"):
+                return float(line.split()[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sim, budget_s):
+    """FP64 oracle on the same state, as many steps as fit in budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    info = sim.info
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    r = sim.positions.astype(np.float64)
+    osys = po.System(N, float(info.R_n), flags=sim.flags, chain_off=sim.chains.astype(np.int32), ff=po.FF_LJ,
+                     csl=float(info.csl), seed=int(info.seed), dt=float(info.dt), bend_all=1, k_b=2.0,
+                     eps_het=0.6, eps_lam=1.0, fcap=float(info.force_cap))
+    step0 = sim.time[2]
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        r = osys.run(r, step0 + steps, 1, prec=64, threads=cores)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": N * steps / dt, "unit": "bead-steps/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 scalar oracle (OpenMP, {cores} threads) on the same N={N} state, "
+                      f"{steps} steps in {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
+              "(independent replicas shard one per GPU with no exchange)", file=sys.stderr)
+        sys.exit(2)
+    import tfm_amd
+
+    print(f"[bench] N={N} R_n={R_N:.3f}: initial condition", file=sys.stderr, flush=True)
+    sim = tfm_amd.Simulation(PARAMS)
+    sim.generate_initial_condition()
+    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    sim.run_steps(args.warmup)
+    sim.synchronize()
+    sim.contact_reset()
+
+    print(f"[bench] timed {args.steps} steps", file=sys.stderr, flush=True)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run_steps(args.steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = N * args.steps / elapsed
+    cmap, n_samples = sim.contact_map()
+
+    # per-kernel device time (HIP events on the simulation's stream)
+    kt = sim.time_kernels(200)
+    force_ms = max(kt[0], kt[2])
+    dominant = "k_force<LJ,1>" if kt[0] >= kt[2] else "k_force<LJ,2>"
+    bytes_force = BYTES_FORCE_PER_BEAD * N
+    achieved = bytes_force / (force_ms * 1e-3) / 1e9
+    traffic = read_traffic(args.traffic)
+    order_bytes = BYTES_ORDER_PER_BEAD * N + BYTES_ORDER_PER_CELL * sim.info.n_c
+    sim.synchronize()
+
+    cpu = None if args.no_cpu else cpu_baseline(sim, args.cpu_seconds)
+    out = {
+        "metric": "bead-steps/s (N=131k, 1 GPU)",
+        "value": value,
+        "unit": "bead-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "ccp BD step, 8 chains x 16384 beads (N=131072), volume fraction 0.3, WCA + het LJ "
+                        "eps 0.6 at 2.5 sigma, LAD 15% at chain ends, dt 1e-3, 2.5 sigma cells, contacts "
+                        "every 1000 steps into 1024x1024 uint32",
+            "N": N, "chains": N_CHAINS, "R_n": round(R_N, 4), "cells_per_side": sim.info.cps,
+            "contact_samples": n_samples, "contact_total": int(cmap.sum(dtype=np.uint64)),
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel_ms": {"force_stage1": float(kt[0]), "order_stage1": float(kt[1]),
+                          "force_stage2": float(kt[2]), "order_stage2": float(kt[3]), "step": float(kt[4])},
+            "bytes_per_launch": {"force": bytes_force, "order": order_bytes},
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    sim.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * ccpmi — MI355X-native Brownian-dynamics engine that drops in for the
+ * simulation path of ccp (Marco-Mendivil-Carboni/TFM, Program/src).
+ *
+ * C ABI: plain pointers and sizes, no exceptions cross it. Every call
+ * returns CCP_OK (0) or an error code; the message of the last error on a
+ * handle is ccp_last_error(h) (ccp_last_error(NULL) for ccp_create). The
+ * messages of parameter errors are ccp's own (chrdat.cu:21-63,
+ * chrsim.cu:428-435), so scripts that grep them keep working.
+ *
+ * One handle = one simulation = one HIP stream on the current device; the
+ * handle owns all its device memory. Calls on one handle are not
+ * thread-safe; distinct handles are independent.
+ *
+ * Reference interfaces replaced (ccp has no FFI; its seams are the C++ class
+ * mmc::chrsim, chrsim.cuh:36-57, and the ccp-perform process boundary,
+ * ccp-perform.cu:43-112). Each function cites the member it stands in for.
+ */
+#ifndef CCPMI_H
+#define CCPMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCP_OK 0
+#define CCP_ERR_PARAM 1 /* invalid parameter (ccp: mmc::error from chrdat/chrsim ctors) */
+#define CCP_ERR_HIP 2   /* HIP runtime failure (ccp: cuda_check, util.cu:83-91) */
+#define CCP_ERR_IO 3    /* file open/read/write failure (ccp: check_file, stream checks) */
+#define CCP_ERR_STATE 4 /* device error flag: cell overflow, bead outside grid, non-finite */
+#define CCP_ERR_ARG 5   /* bad argument to this API (size mismatch, NULL) */
+
+/* Per-bead flag bits. ccp's ptype LADh ("A") is CCP_HET|CCP_LAD, LNDe ("B") is 0. */
+#define CCP_HET 1u
+#define CCP_LAD 2u
+
+/* Force fields. CCP is ccp's default simulation type (DST); LJ is the
+ * optional WCA/LJ/lamina-well set selected by "force_field lj". */
+#define CCP_FF_CCP 0
+#define CCP_FF_ICG 1
+#define CCP_FF_LJ 2
+
+typedef struct ccp_sim ccp_sim;
+
+typedef struct ccp_info {
+  uint32_t N;           /* beads */
+  uint32_t n_l;         /* lamina binding sites */
+  uint32_t n_chains;
+  uint32_t force_field; /* CCP_FF_CCP or CCP_FF_LJ */
+  uint32_t cps;         /* cells per side */
+  uint32_t n_c;         /* cells = cps^3 */
+  uint32_t cell_capacity;
+  uint32_t spf;         /* steps per frame */
+  uint32_t fpf;         /* frames per file */
+  uint32_t contact_bins;     /* contact map side */
+  uint32_t contact_bin_size; /* beads per contact bin */
+  uint32_t contact_interval; /* steps between contact samples (0 = off) */
+  float csl;            /* cell side length */
+  float R_n, R_o, R_b;
+  float T, dt, sd;      /* temperature, timestep, noise standard deviation */
+  float contact_cutoff;
+  float force_cap;      /* integrator force cap (0 = off; the LJ set's safeguard) */
+  uint64_t seed;
+} ccp_info;
+
+/* Logger callback: one line per call (no timestamp; the callee adds it). */
+typedef void (*ccp_log_fn)(const char *line, void *user);
+
+/* Create a simulation from the text of adjustable-parameters.dat
+ * (whitespace-separated "key value" pairs; unknown keys ignored, first
+ * occurrence wins). Replaces chrsim::chrsim(parmap&) (chrsim.cu:421-455)
+ * with chrdat::chrdat (chrdat.cu:40-88), cngeom (chrdat.cu:13-37) and
+ * parmap (util.cu:72-80). Returns NULL on error. */
+ccp_sim *ccp_create(const char *param_text);
+
+/* ccp_create with the logger installed before the parameter echo lines
+ * (ccp-perform sets its log file before constructing chrsim,
+ * ccp-perform.cu:34-48). */
+ccp_sim *ccp_create_logged(const char *param_text, ccp_log_fn fn, void *user);
+
+/* Release device and host memory (~chrsim/~chrdat, chrsim.cu:458-467). */
+void ccp_destroy(ccp_sim *h);
+
+/* Message of the last error on h, or of the last failed ccp_create when h is NULL. */
+const char *ccp_last_error(const ccp_sim *h);
+
+/* Route log lines (parameter echo, progress, events) to fn instead of
+ * stdout (mmc::logger, util.cu:15-50). fn NULL restores stdout. */
+int ccp_set_logger(ccp_sim *h, ccp_log_fn fn, void *user);
+
+int ccp_get_info(const ccp_sim *h, ccp_info *out);
+
+/* Random LBS sites, bead types, confined random walk and overlap removal
+ * (chrsim::generate_initial_condition, chrsim.cu:470-522, with
+ * set_lbs_positions :596-631, set_particle_types :634-675,
+ * set_particle_positons :678-716, perform_random_walk :719-802,
+ * particle_overlaps :805-820). */
+int ccp_generate_initial_condition(ccp_sim *h);
+
+/* chrsim::save_checkpoint / load_checkpoint (chrsim.cu:525-555): i_f u32,
+ * t f32, types N x i32, r N x 3 f32, lbs n_l x 3 f32, then a trailer that
+ * ccp ignores (magic, seed, step, flags) so resumes continue the Philox
+ * stream bit-exactly. Files without the trailer load as in ccp. */
+int ccp_save_checkpoint(ccp_sim *h, const char *path);
+int ccp_load_checkpoint(ccp_sim *h, const char *path);
+
+/* .gro text frame and LBS file (chrdat::write_frame_txt :107-133,
+ * read_frame_txt :136-165, write_lbs_txt :229-254, read_lbs_txt :257-284). */
+int ccp_write_frame_txt(ccp_sim *h, const char *path);
+int ccp_read_frame_txt(ccp_sim *h, const char *path);
+int ccp_write_lbs_txt(ccp_sim *h, const char *path);
+int ccp_read_lbs_txt(ccp_sim *h, const char *path);
+
+/* chrsim::run_simulation (chrsim.cu:558-593): fpf frames of spf steps; each
+ * frame is appended to the minimal .trr file at trr_path (truncated first,
+ * chrdat::write_frame_bin :168-201). trr_path NULL runs without output. */
+int ccp_run_simulation(ccp_sim *h, const char *trr_path);
+
+/* The step loop alone (chrsim.cu:570-580): n stochastic-Heun steps with
+ * contact sampling every contact_interval steps; no host transfer. */
+int ccp_run_steps(ccp_sim *h, uint64_t n);
+
+/* Wait for the handle's stream and report device error flags. */
+int ccp_synchronize(ccp_sim *h);
+
+/* Frame index, time and global step counter (chrdat i_f, t). */
+int ccp_get_time(const ccp_sim *h, uint32_t *i_f, float *t, uint64_t *step);
+int ccp_set_time(ccp_sim *h, uint32_t i_f, float t, uint64_t step);
+
+/* Host copies of the state (xyz interleaved floats; n = count of beads or
+ * sites). Types use ccp's ptype codes: 0 = LADh (A), 1 = LNDe (B). */
+int ccp_get_positions(ccp_sim *h, float *xyz, size_t n);
+int ccp_set_positions(ccp_sim *h, const float *xyz, size_t n);
+int ccp_get_types(ccp_sim *h, int32_t *pt, size_t n);
+int ccp_set_types(ccp_sim *h, const int32_t *pt, size_t n);
+int ccp_get_flags(ccp_sim *h, uint8_t *flags, size_t n);
+int ccp_set_flags(ccp_sim *h, const uint8_t *flags, size_t n);
+int ccp_get_lbs(ccp_sim *h, float *xyz, size_t n_l);
+int ccp_set_lbs(ccp_sim *h, const float *xyz, size_t n_l);
+/* chain offsets, n_chains + 1 entries */
+int ccp_get_chains(ccp_sim *h, uint32_t *off, size_t n);
+
+/* One force evaluation F(r) of the current positions, as exec_RK_1 computes
+ * it (calc_bf + calc_cf + calc_srf, chrsim.cu:381-384); ff selects
+ * CCP_FF_CCP/LJ (the handle's own) or CCP_FF_ICG. */
+int ccp_compute_forces(ccp_sim *h, int ff, float *f_xyz, size_t n);
+
+/* The cell list of the current positions in ccp's layout
+ * (sugrid::generate_arrays, sugrid.cu:125-133): sci/soi sorted keys and
+ * bead indices (n = N), beg/end per cell (n_c entries, empty cells 0/0). */
+int ccp_cell_list(ccp_sim *h, uint32_t *sci, uint32_t *soi, uint32_t *beg, uint32_t *end, size_t n, size_t n_c);
+
+/* The noise of step `step` for every bead, in units of one standard
+ * deviation (begin_iter, chrsim.cu:348-356), and the redraw count. */
+int ccp_noise(ccp_sim *h, uint64_t step, float *z_xyz, uint32_t *attempts, size_t n);
+
+/* Count non-bonded bead pairs (|i-j| > 1) closer than dmin
+ * (chrsim::particle_overlaps, chrsim.cu:805-820, on the device). */
+int ccp_count_overlaps(ccp_sim *h, float dmin, uint64_t *count);
+
+/* Contact map: ordered pairs i != j with |r_i - r_j| < contact_cutoff,
+ * binned by bead index (bin = i / contact_bin_size) into a
+ * contact_bins^2 uint32 map accumulated with integer atomics. */
+int ccp_contact_sample(ccp_sim *h);
+int ccp_contact_reset(ccp_sim *h);
+int ccp_contact_map(ccp_sim *h, uint32_t *map, size_t n_bins2, uint32_t *n_samples);
+
+/* Average device time per launch (ms) of each kernel of the step over
+ * n_steps steps, measured with HIP events on the handle's stream:
+ * out[0] force stage 1, [1] order (scan + sort + gather) after stage 1,
+ * [2] force stage 2, [3] order after stage 2, [4] whole step. */
+int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,138 @@
+"""GPU parity: the HIP path against the CPU oracle on identical inputs."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from gpu_common import make_pair, step
+from helpers import pattern, radius_for, relaxed, walk
+
+pytestmark = pytest.mark.gpu
+
+
+def lbs_sites(R_n, n, seed):
+    d = np.random.default_rng(seed).normal(size=(n, 3))
+    return (R_n - po.CCP_CONST["rco"]) * d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+@pytest.mark.parametrize("N,cvf,seed", [(1, 0.2, 1), (1000, 0.2, 3), (16384, 0.2, 4), (131072, 0.3, 5)])
+def test_cell_list_bit_exact(N, cvf, seed):
+    step(f"cell list N={N}")
+    R_n = radius_for(N, cvf) if N > 1 else 5.0
+    r = walk(N, R_n, seed)
+    sim, osys, r64 = make_pair(N, R_n, np.zeros(N, np.uint8), r)
+    got = sim.cell_list()
+    want = po.cell_list(r64.astype(np.float32), np.float32(2.0), sim.info.cps)
+    assert want["bad"] == 0
+    for k in ("sci", "soi", "beg", "end"):
+        assert np.array_equal(got[k], want[k]), k
+    sim.close()
+
+
+def test_noise_matches_oracle():
+    step("noise")
+    N = 200_000
+    sim, _, _ = make_pair(N, radius_for(N, 0.2), np.zeros(N, np.uint8), np.zeros((N, 3)), seed=0xDEADBEEF12345)
+    for s in (0, 7, (3 << 32) + 5):
+        z, a = sim.noise(s)
+        zo, ao = po.noise_batch(0xDEADBEEF12345, s, N)
+        assert np.array_equal(a, ao.astype(np.uint32))
+        assert np.abs(z - zo).max() < 2e-5
+    assert (a > 0).sum() <= 10
+    sim.close()
+
+
+FORCE_CASES = [
+    dict(ff=po.FF_CCP, n_l=40),
+    dict(ff=po.FF_ICG, n_l=0),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2),
+    dict(ff=po.FF_CCP, n_l=20, opening_radius=4.0, bleb_radius=6.0),
+]
+
+
+@pytest.mark.parametrize("case", FORCE_CASES)
+def test_forces_match_oracle(case):
+    case = dict(case)
+    step(f"forces {case}")
+    N = 2000
+    R_n = radius_for(N, 0.3)
+    r = relaxed(N, R_n, 7, chain_off=[0, 700, 1400, 2000])
+    r *= 1.0 + 0.3 / R_n
+    n_l = case.pop("n_l")
+    lbs = lbs_sites(R_n, n_l, 3) if n_l else None
+    ff = case.pop("ff")
+    fl = pattern(N, 2)
+    if ff == po.FF_LJ:
+        rng = np.random.default_rng(6)
+        fl = ((rng.random(N) < 0.3) * po.HET | (rng.random(N) < 0.15) * po.LAD).astype(np.uint8)
+    sim_ff = po.FF_CCP if ff == po.FF_ICG else ff
+    sim, osys, r64 = make_pair(N, R_n, fl, r, ff=sim_ff, lbs=lbs, chain_off=[0, 700, 1400, 2000], **case)
+    osys.p.ff = ff
+    got = sim.compute_forces(ff).astype(np.float64)
+    want = osys.forces(r64, method=0)
+    w32 = osys.forces(r64, method=0, prec=32)
+    scale = np.abs(want).max()
+    err = np.abs(got - want).max()
+    err32 = np.abs(w32 - want).max()
+    step(f"  max|f|={scale:.3g} gpu err={err:.3g} oracle fp32 err={err32:.3g}")
+    assert err <= 1e-5 * scale + 1e-4
+    sim.close()
+
+
+@pytest.mark.parametrize("ff,dt,tol", [(po.FF_CCP, 1 / 2048, 1e-3), (po.FF_LJ, 1e-3, 2e-3)])
+def test_trajectory_1000_steps(ff, dt, tol):
+    step(f"trajectory ff={ff}")
+    N = 1024
+    R_n = radius_for(N, 0.2)
+    r = relaxed(N, R_n, 2)
+    if ff == po.FF_LJ:
+        rng = np.random.default_rng(4)
+        fl = ((rng.random(N) < 0.3) * po.HET | (rng.random(N) < 0.1) * po.LAD).astype(np.uint8)
+        lbs = None
+    else:
+        fl = pattern(N, 1)
+        lbs = lbs_sites(R_n, 30, 8)
+    sim, osys, r64 = make_pair(N, R_n, fl, r, ff=ff, dt=dt, lbs=lbs)
+    sim.set_time(0, 0.0, 0)
+    sim.run_steps(1000)
+    sim.synchronize()
+    got = sim.positions.astype(np.float64)
+    want = osys.run(r64, 0, 1000, prec=64)
+    dev = np.linalg.norm(got - want, axis=1)
+    moved = np.linalg.norm(want - r64, axis=1)
+    step(f"  max dev={dev.max():.3g} p99={np.percentile(dev, 99):.3g} mean displacement={moved.mean():.3g}")
+    assert dev.max() < tol
+    assert moved.mean() > 50 * tol
+    assert sim.time[2] == 1000
+    sim.close()
+
+
+@pytest.mark.parametrize("N,bs,rc,ff", [(1024, 1, 2.0, po.FF_CCP), (16384, 16, 1.5, po.FF_LJ),
+                                        (131072, 128, 1.5, po.FF_LJ)])
+def test_contacts_bit_exact(N, bs, rc, ff):
+    step(f"contacts N={N}")
+    R_n = radius_for(N, 0.3)
+    r = walk(N, R_n, 11)
+    sim, _, r64 = make_pair(N, R_n, np.zeros(N, np.uint8), r, ff=ff)
+    assert sim.info.contact_bin_size == bs and sim.info.contact_cutoff == pytest.approx(rc)
+    sim.contact_reset()
+    sim.contact_sample()
+    sim.contact_sample()
+    m, ns = sim.contact_map()
+    want = po.contacts_grid(r64.astype(np.float32), np.float32(rc), bs, sim.info.contact_bins)
+    assert ns == 2
+    assert np.array_equal(m, 2 * want)
+    assert np.array_equal(m, m.T)
+    sim.close()
+
+
+def test_overlap_count_matches_brute_force():
+    step("overlaps")
+    N = 3000
+    R_n = radius_for(N, 0.3)
+    r = walk(N, R_n, 12)
+    sim, _, r64 = make_pair(N, R_n, np.zeros(N, np.uint8), r)
+    d = np.linalg.norm(r64[:, None, :] - r64[None, :, :], axis=2)
+    iu = np.triu_indices(N, k=2)
+    want = int((d[iu] < 0.838732).sum())
+    assert sim.count_overlaps(0.838732) == want
+    sim.close()

@@ -1,0 +1,288 @@
+"""Python mirror of ccp's simulation class over the libccpmi C ABI.
+
+`Simulation` stands in for mmc::chrsim (Program/src/chrsim.cuh:36-57): it is
+built from the adjustable-parameters text, generates initial conditions,
+saves/loads checkpoints, writes ccp's .gro/.trr files and runs the step on
+the GPU. Errors raise CcpError carrying ccp's own messages (chrdat.cu:21-63).
+There is no CPU path: without the built HIP library every call fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libccpmi.so")
+
+CCP_OK, CCP_ERR_PARAM, CCP_ERR_HIP, CCP_ERR_IO, CCP_ERR_STATE, CCP_ERR_ARG = range(6)
+HET, LAD = 1, 2
+FF_CCP, FF_ICG, FF_LJ = 0, 1, 2
+
+# Every symbol declared in include/ccpmi.h.
+EXPORTS = (
+    "ccp_create", "ccp_create_logged", "ccp_destroy", "ccp_last_error", "ccp_set_logger", "ccp_get_info",
+    "ccp_generate_initial_condition", "ccp_save_checkpoint", "ccp_load_checkpoint", "ccp_write_frame_txt",
+    "ccp_read_frame_txt", "ccp_write_lbs_txt", "ccp_read_lbs_txt", "ccp_run_simulation", "ccp_run_steps",
+    "ccp_synchronize", "ccp_get_time", "ccp_set_time", "ccp_get_positions", "ccp_set_positions",
+    "ccp_get_types", "ccp_set_types", "ccp_get_flags", "ccp_set_flags", "ccp_get_lbs", "ccp_set_lbs",
+    "ccp_get_chains", "ccp_compute_forces", "ccp_cell_list", "ccp_noise", "ccp_count_overlaps",
+    "ccp_contact_sample", "ccp_contact_reset", "ccp_contact_map", "ccp_time_kernels",
+)
+
+
+class CcpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class Info(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "N", "n_l", "n_chains", "force_field", "cps", "n_c", "cell_capacity", "spf", "fpf",
+        "contact_bins", "contact_bin_size", "contact_interval")] + \
+        [(n, ctypes.c_float) for n in ("csl", "R_n", "R_o", "R_b", "T", "dt", "sd", "contact_cutoff",
+                                             "force_cap")] + \
+        [("seed", ctypes.c_uint64)]
+
+
+LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
+
+_lib = None
+
+
+def lib():
+    """Load libccpmi.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m tfm_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        f32p, u32p, i32p, u8p = P(ctypes.c_float), P(ctypes.c_uint32), P(ctypes.c_int32), P(ctypes.c_uint8)
+        L.ccp_create.argtypes = [ctypes.c_char_p]
+        L.ccp_create.restype = vp
+        L.ccp_create_logged.argtypes = [ctypes.c_char_p, LOG_FN, vp]
+        L.ccp_create_logged.restype = vp
+        L.ccp_destroy.argtypes = [vp]
+        L.ccp_last_error.argtypes = [vp]
+        L.ccp_last_error.restype = ctypes.c_char_p
+        L.ccp_set_logger.argtypes = [vp, LOG_FN, vp]
+        L.ccp_get_info.argtypes = [vp, P(Info)]
+        for n in ("ccp_generate_initial_condition", "ccp_synchronize", "ccp_contact_sample", "ccp_contact_reset"):
+            getattr(L, n).argtypes = [vp]
+        for n in ("ccp_save_checkpoint", "ccp_load_checkpoint", "ccp_write_frame_txt", "ccp_read_frame_txt",
+                  "ccp_write_lbs_txt", "ccp_read_lbs_txt", "ccp_run_simulation"):
+            getattr(L, n).argtypes = [vp, ctypes.c_char_p]
+        L.ccp_run_steps.argtypes = [vp, ctypes.c_uint64]
+        L.ccp_get_time.argtypes = [vp, u32p, f32p, P(ctypes.c_uint64)]
+        L.ccp_set_time.argtypes = [vp, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64]
+        L.ccp_get_positions.argtypes = [vp, f32p, sz]
+        L.ccp_set_positions.argtypes = [vp, f32p, sz]
+        L.ccp_get_types.argtypes = [vp, i32p, sz]
+        L.ccp_set_types.argtypes = [vp, i32p, sz]
+        L.ccp_get_flags.argtypes = [vp, u8p, sz]
+        L.ccp_set_flags.argtypes = [vp, u8p, sz]
+        L.ccp_get_lbs.argtypes = [vp, f32p, sz]
+        L.ccp_set_lbs.argtypes = [vp, f32p, sz]
+        L.ccp_get_chains.argtypes = [vp, u32p, sz]
+        L.ccp_compute_forces.argtypes = [vp, ctypes.c_int, f32p, sz]
+        L.ccp_cell_list.argtypes = [vp, u32p, u32p, u32p, u32p, sz, sz]
+        L.ccp_noise.argtypes = [vp, ctypes.c_uint64, f32p, u32p, sz]
+        L.ccp_count_overlaps.argtypes = [vp, ctypes.c_float, P(ctypes.c_uint64)]
+        L.ccp_contact_map.argtypes = [vp, u32p, sz, u32p]
+        L.ccp_time_kernels.argtypes = [vp, ctypes.c_uint32, f32p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def params_text(params):
+    if isinstance(params, str):
+        return params
+    return "".join(f"{k} {v}\n" for k, v in params.items())
+
+
+def _quiet(line, user):
+    pass
+
+
+_QUIET = LOG_FN(_quiet)
+
+
+class Simulation:
+    """One simulation on the current GPU (ccp's chrsim)."""
+
+    def __init__(self, params, quiet=True):
+        L = lib()
+        text = params_text(params).encode()
+        self._log_keep = _QUIET if quiet else None
+        h = L.ccp_create_logged(text, _QUIET, None) if quiet else L.ccp_create(text)
+        if not h:
+            msg = L.ccp_last_error(None).decode()
+            code = CCP_ERR_HIP if msg.startswith("hip:") else CCP_ERR_PARAM
+            raise CcpError(code, msg)
+        self.h = ctypes.c_void_p(h)
+        self.info = Info()
+        L.ccp_get_info(self.h, ctypes.byref(self.info))
+        self.N = self.info.N
+        self.n_l = self.info.n_l
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ccp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != CCP_OK:
+            raise CcpError(rc, lib().ccp_last_error(self.h).decode())
+
+    # chrsim members
+    def generate_initial_condition(self):
+        self._check(lib().ccp_generate_initial_condition(self.h))
+
+    def save_checkpoint(self, path):
+        self._check(lib().ccp_save_checkpoint(self.h, str(path).encode()))
+
+    def load_checkpoint(self, path):
+        self._check(lib().ccp_load_checkpoint(self.h, str(path).encode()))
+
+    def run_simulation(self, path=None):
+        self._check(lib().ccp_run_simulation(self.h, None if path is None else str(path).encode()))
+
+    def write_frame_txt(self, path):
+        self._check(lib().ccp_write_frame_txt(self.h, str(path).encode()))
+
+    def read_frame_txt(self, path):
+        self._check(lib().ccp_read_frame_txt(self.h, str(path).encode()))
+
+    def write_lbs_txt(self, path):
+        self._check(lib().ccp_write_lbs_txt(self.h, str(path).encode()))
+
+    def read_lbs_txt(self, path):
+        self._check(lib().ccp_read_lbs_txt(self.h, str(path).encode()))
+
+    # the step
+    def run_steps(self, n):
+        self._check(lib().ccp_run_steps(self.h, int(n)))
+
+    def synchronize(self):
+        self._check(lib().ccp_synchronize(self.h))
+
+    # state
+    @property
+    def time(self):
+        i_f, t, s = ctypes.c_uint32(), ctypes.c_float(), ctypes.c_uint64()
+        lib().ccp_get_time(self.h, ctypes.byref(i_f), ctypes.byref(t), ctypes.byref(s))
+        return i_f.value, t.value, s.value
+
+    def set_time(self, i_f, t, step):
+        self._check(lib().ccp_set_time(self.h, i_f, t, step))
+
+    @property
+    def positions(self):
+        out = np.zeros((self.N, 3), np.float32)
+        self._check(lib().ccp_get_positions(self.h, _ptr(out, ctypes.c_float), self.N))
+        return out
+
+    @positions.setter
+    def positions(self, xyz):
+        a = np.ascontiguousarray(xyz, dtype=np.float32).reshape(self.N, 3)
+        self._check(lib().ccp_set_positions(self.h, _ptr(a, ctypes.c_float), self.N))
+
+    @property
+    def types(self):
+        out = np.zeros(self.N, np.int32)
+        self._check(lib().ccp_get_types(self.h, _ptr(out, ctypes.c_int32), self.N))
+        return out
+
+    @types.setter
+    def types(self, pt):
+        a = np.ascontiguousarray(pt, dtype=np.int32)
+        self._check(lib().ccp_set_types(self.h, _ptr(a, ctypes.c_int32), self.N))
+
+    @property
+    def flags(self):
+        out = np.zeros(self.N, np.uint8)
+        self._check(lib().ccp_get_flags(self.h, _ptr(out, ctypes.c_uint8), self.N))
+        return out
+
+    @flags.setter
+    def flags(self, fl):
+        a = np.ascontiguousarray(fl, dtype=np.uint8)
+        self._check(lib().ccp_set_flags(self.h, _ptr(a, ctypes.c_uint8), self.N))
+
+    @property
+    def lbs(self):
+        out = np.zeros((self.n_l, 3), np.float32)
+        self._check(lib().ccp_get_lbs(self.h, _ptr(out, ctypes.c_float), self.n_l))
+        return out
+
+    @lbs.setter
+    def lbs(self, xyz):
+        a = np.ascontiguousarray(xyz, dtype=np.float32).reshape(self.n_l, 3)
+        self._check(lib().ccp_set_lbs(self.h, _ptr(a, ctypes.c_float), self.n_l))
+
+    @property
+    def chains(self):
+        out = np.zeros(self.info.n_chains + 1, np.uint32)
+        self._check(lib().ccp_get_chains(self.h, _ptr(out, ctypes.c_uint32), out.size))
+        return out
+
+    # parity / measurement access
+    def compute_forces(self, ff=None):
+        ff = self.info.force_field if ff is None else ff
+        out = np.zeros((self.N, 3), np.float32)
+        self._check(lib().ccp_compute_forces(self.h, ff, _ptr(out, ctypes.c_float), self.N))
+        return out
+
+    def cell_list(self):
+        n, n_c = self.N, self.info.n_c
+        sci, soi = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        beg, end = np.zeros(n_c, np.uint32), np.zeros(n_c, np.uint32)
+        self._check(lib().ccp_cell_list(self.h, *(_ptr(a, ctypes.c_uint32) for a in (sci, soi, beg, end)), n, n_c))
+        return dict(sci=sci, soi=soi, beg=beg, end=end)
+
+    def noise(self, step):
+        z = np.zeros((self.N, 3), np.float32)
+        a = np.zeros(self.N, np.uint32)
+        self._check(lib().ccp_noise(self.h, step, _ptr(z, ctypes.c_float), _ptr(a, ctypes.c_uint32), self.N))
+        return z, a
+
+    def count_overlaps(self, dmin):
+        c = ctypes.c_uint64()
+        self._check(lib().ccp_count_overlaps(self.h, dmin, ctypes.byref(c)))
+        return c.value
+
+    def contact_sample(self):
+        self._check(lib().ccp_contact_sample(self.h))
+
+    def contact_reset(self):
+        self._check(lib().ccp_contact_reset(self.h))
+
+    def contact_map(self):
+        b = self.info.contact_bins
+        m = np.zeros((b, b), np.uint32)
+        ns = ctypes.c_uint32()
+        self._check(lib().ccp_contact_map(self.h, _ptr(m, ctypes.c_uint32), b * b, ctypes.byref(ns)))
+        return m, ns.value
+
+    def time_kernels(self, n_steps):
+        out = np.zeros(5, np.float32)
+        self._check(lib().ccp_time_kernels(self.h, n_steps, _ptr(out, ctypes.c_float)))
+        return out

@@ -1,0 +1,1402 @@
+// Host side of libccpmi: the C ABI of include/ccpmi.h.
+//
+// Parameters, validation messages and file formats follow ccp
+// (Program/src/chrdat.cu, chrsim.cu, util.cu); the step runs on the gfx950
+// kernels of kernels.hip, captured into hipGraphs of `graph_steps` steps.
+
+#include "ccpmi.h"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace ccpmi;
+
+namespace {
+
+// ---------------------------------------------------------------- constants
+// ccp model constants (chrdat.cuh:15-24, chrsim.cuh:18-20)
+constexpr float K_B = 0.003356f;
+constexpr float L_0 = 1.0f;
+constexpr float K_E = 128.0f;
+constexpr float K_BEND = 1.0f;
+constexpr float E_P = 0.5f;
+constexpr float RCO = 1.154701f;
+constexpr float ACO = 2.0f;
+constexpr float MDL = 21.10139f;
+constexpr float E_L = 8.0f;
+constexpr float LCO = 0.5f;
+constexpr float DT_DEF = 1.0f / 2048;
+constexpr float MIS = 0.838732f;
+constexpr uint32_t N_DEF = 30386;
+const uint32_t CHR_LIM[13] = {0, 2284, 4599, 6687, 9338, 12508, 15193, 17477, 19792, 21880, 24531, 27701, 30386};
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e) {
+  if (e != hipSuccess) throw Error(CCP_ERR_HIP, std::string("hip: ") + hipGetErrorString(e));
+}
+
+// whitespace "key value" pairs; the first occurrence of a key wins (util.cu:72-80)
+struct ParMap : std::map<std::string, std::string> {
+  explicit ParMap(const char *text) {
+    std::istringstream is(text ? text : "");
+    std::string k, v;
+    while (is >> k >> v) insert({k, v});
+  }
+  template <class T> T get(const std::string &key, T def) const {
+    auto it = find(key);
+    if (it == end()) return def;
+    T val{};
+    std::stringstream{it->second} >> val;
+    return val;
+  }
+  bool has(const std::string &k) const { return find(k) != end(); }
+};
+
+// cnfs (util.cuh:91-104): formatted number
+template <class T> std::string cnfs(T num, int len = 0, char fillc = ' ', int prc = 0) {
+  std::stringstream s;
+  if (len > 0) s << std::setw(len);
+  if (fillc != ' ') s << std::setfill(fillc);
+  if (prc > 0) s << std::setprecision(prc) << std::fixed;
+  s << num;
+  return s.str();
+}
+
+// ---------------------------------------------------------------- host Philox stream
+// Uniforms for the initial condition: Philox4x32-10 with the simulation seed
+// as key and counter {n lo, n hi, 0xffffffff, 1 << 16}, disjoint from the
+// step-noise counters {bead, step lo, step hi, attempt < 2^16}.
+struct HostRng {
+  uint32_t k0, k1;
+  uint64_t n = 0;
+  uint32_t buf[4];
+  int pos = 4;
+  explicit HostRng(uint64_t seed) : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)) {}
+  void refill() {
+    uint32_t c[4] = {(uint32_t)n, (uint32_t)(n >> 32), 0xffffffffu, 1u << 16};
+    uint32_t a = k0, b = k1;
+    for (int r = 0; r < 10; ++r) {
+      uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+      uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ a, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ b;
+      c[1] = (uint32_t)p1;
+      c[3] = (uint32_t)p0;
+      c[0] = n0;
+      c[2] = n2;
+      a += 0x9E3779B9u;
+      b += 0xBB67AE85u;
+    }
+    std::memcpy(buf, c, sizeof(c));
+    ++n;
+    pos = 0;
+  }
+  // uniform in (0, 1] like curandGenerateUniform
+  float uniform() {
+    if (pos == 4) refill();
+    return (float)((buf[pos++] >> 8) + 1u) * (1.0f / 16777216.0f);
+  }
+};
+
+// inverse error function: Winitzki's closed-form start (2008) refined by
+// Newton steps on std::erf
+double erfinv(double x) {
+  if (x <= -1.0) return -INFINITY;
+  if (x >= 1.0) return INFINITY;
+  const double a = 0.147, ln = std::log(1.0 - x * x), t = 2.0 / (M_PI * a) + ln / 2.0;
+  double y = std::copysign(std::sqrt(std::sqrt(t * t - ln / a) - t), x);
+  for (int k = 0; k < 4; ++k) y -= (std::erf(y) - x) / (2.0 / std::sqrt(M_PI) * std::exp(-y * y));
+  return y;
+}
+
+struct V3 {
+  float x, y, z;
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+template <class T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    n = count;
+    hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+  }
+  void free_() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+};
+
+} // namespace
+
+// ---------------------------------------------------------------- handle
+struct ccp_sim {
+  // parameters
+  uint32_t N = 0, n_l = 0, fpf = 0, spf = 0, tpb = 0;
+  float T = 0, R_n = 0, R_o = 0, R_b = 0;
+  float noc = 0, boc = 0, nod = 0, bod = 0, d_m = 0;
+  float dt = 0, sd = 0;
+  uint64_t seed = 0;
+  int ff = CCP_FF_CCP;
+  float csl = 0;
+  uint32_t cps = 0, n_c = 0, cap = 0, nblk = 0;
+  std::vector<uint32_t> chain_off;
+  std::string seq_file;
+  int seq_mode = 0; // 0 ccp, 1 het/LAD generator
+  float het_fraction = 0.3f, het_mean = 20.0f, lad_fraction = 0.1f;
+  int lad_ends = 1;
+  float k_b = K_BEND;
+  int bend_all = 0, bond_type = 0;
+  float eps_het = 0.5f, eps_lam = 1.0f;
+  float contact_rc = 0;
+  uint32_t cbins = 0, cbin_size = 0, cinterval = 0;
+  uint32_t graph_steps = 64;
+  float fcap = 0.0f;
+  KParams kp{};
+
+  // state
+  uint32_t i_f = 0;
+  float t = 0.0f;
+  uint64_t step = 0;
+  std::vector<float> hr, hlr;
+  std::vector<uint8_t> hfl;
+  bool grid_valid = false, lgrid_valid = false, dstep_dirty = true;
+  int clean_bs = 0;
+  uint32_t n_samples = 0;
+
+  // device
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  DevBuf<float4> r, er, pstar, spos, lr, lspos, aux;
+  DevBuf<uint32_t> cstart, lstart, count, bs0, bs1, bucket, soi, flags, cmap;
+  DevBuf<uint64_t> dstep;
+  DevBuf<unsigned long long> dcount;
+  hipGraphExec_t graphs[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+
+  // misc
+  std::string err;
+  ccp_log_fn log_fn = nullptr;
+  void *log_user = nullptr;
+
+  uint32_t *bs(int k) { return k ? bs1.p : bs0.p; }
+
+  ~ccp_sim() {
+    for (auto &gm : graphs)
+      for (auto &g : gm)
+        if (g) (void)hipGraphExecDestroy(g);
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &spos, &lr, &lspos, &aux};
+    for (auto *b : f4) b->free_();
+    DevBuf<uint32_t> *u4[] = {&cstart, &lstart, &count, &bs0, &bs1, &bucket, &soi, &flags, &cmap};
+    for (auto *b : u4) b->free_();
+    dstep.free_();
+    dcount.free_();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // -------------------------------------------------------------- logging
+  void record(const std::string &msg) {
+    if (log_fn) {
+      log_fn(msg.c_str(), log_user);
+      return;
+    }
+    time_t now = time(nullptr);
+    char ts[21];
+    strftime(ts, 21, "[%d/%m/%y %H:%M:%S] ", localtime(&now));
+    std::cout << ts << msg << std::endl;
+  }
+  void progress(float pc) {
+    std::string line = "progress: " + cnfs(pc, 5, '0', 1) + "%";
+    if (log_fn) {
+      log_fn(line.c_str(), log_user);
+      return;
+    }
+    std::cout << line << "\r";
+    std::cout.flush();
+  }
+
+  // -------------------------------------------------------------- setup
+  void configure(const ParMap &par);
+  void allocate();
+  void fill_kparams();
+  uint32_t wbits(uint32_t i) const;
+  void upload_positions();
+  void upload_lbs();
+  void download_positions();
+  void check_flags();
+  // -------------------------------------------------------------- step machinery
+  KParams mode_params(int mode) const {
+    KParams k = kp;
+    if (mode == MODE_ICG) k.sd = 0.0f;
+    return k;
+  }
+  void build_grid(const float4 *pos, int n_obj, float4 *out_spos, uint32_t *out_cstart, uint32_t *out_soi);
+  void ensure_grids();
+  void sync_step();
+  void launch_stage(int mode, int stage);
+  void launch_step(int mode) {
+    launch_stage(mode, 1);
+    launch_stage(mode, 2);
+  }
+  void run_steps(uint64_t n, int mode);
+  void run_chunk(uint64_t n, int mode);
+  // -------------------------------------------------------------- initial condition
+  void set_lbs_positions(HostRng &g);
+  void set_particle_types(HostRng &g);
+  void set_particle_positions(HostRng &g);
+  void perform_random_walk(HostRng &g, uint32_t i_s, uint32_t i_e, V3 dir, float mda);
+  uint64_t count_overlaps(float dmin);
+  void generate_initial_condition();
+  // -------------------------------------------------------------- files
+  void write_frame_bin(std::ofstream &f);
+};
+
+void ccp_sim::configure(const ParMap &par) {
+  // chrdat::chrdat and cngeom (chrdat.cu:13-75), chrsim::chrsim (chrsim.cu:421-439)
+  N = par.get<uint32_t>("number_of_particles", N_DEF);
+  R_n = par.get<float>("nucleus_radius", 0.0f);
+  R_o = par.get<float>("opening_radius", 0.0f);
+  R_b = par.get<float>("bleb_radius", 0.0f);
+  noc = (float)std::cos(std::asin((double)R_o / R_n));
+  boc = (float)std::cos(std::asin((double)R_o / R_b));
+  nod = (float)std::sqrt((double)R_n * R_n - (double)R_o * R_o);
+  bod = (float)std::sqrt((double)R_b * R_b - (double)R_o * R_o);
+  d_m = (nod + bod) + R_b;
+  if (!(0.0f < R_n && R_n < 1000.0f)) throw Error(CCP_ERR_PARAM, "nucleus_radius out of range");
+  if (!(0.0f <= R_o && R_o < R_n)) throw Error(CCP_ERR_PARAM, "opening_radius out of range");
+  if (!(R_o <= R_b && R_b < R_n)) throw Error(CCP_ERR_PARAM, "bleb_radius out of range");
+  record("R_n = " + cnfs(R_n, 5, '0', 2) + " R_o = " + cnfs(R_o, 5, '0', 2) + " R_b = " + cnfs(R_b, 5, '0', 2) +
+         " ");
+
+  T = par.get<float>("temperature", 298.0f);
+  n_l = par.get<uint32_t>("number_of_lbs", 0u);
+  fpf = par.get<uint32_t>("frames_per_file", 128u);
+  if (!(1 <= N && N < (1u << ID_BITS))) throw Error(CCP_ERR_PARAM, "number_of_particles out of range");
+  if (!(0.0f <= T && T < 1000.0f)) throw Error(CCP_ERR_PARAM, "temperature out of range");
+  if (!(n_l < 100000)) throw Error(CCP_ERR_PARAM, "number_of_lbs out of range");
+  if (!(1 <= fpf && fpf < 10000)) throw Error(CCP_ERR_PARAM, "frames_per_file out of range");
+  float cvf = (float)(N * std::pow((RCO / 2.0) / (R_n - (RCO / 2.0)), 3.0));
+  if (cvf > 0.4f) throw Error(CCP_ERR_PARAM, "chromatin volume fraction above 0.4");
+  float noacf = (float)(2.0 / (1.0 + noc));
+  float laf = (float)(noacf * n_l * std::pow(LCO / (R_n - RCO), 2.0) / 4.0);
+  if (laf > 0.4f) throw Error(CCP_ERR_PARAM, "lbs area fraction above 0.4");
+  record("N = " + cnfs(N, 5, '0') + " T = " + cnfs(T, 5, '0', 1) + " n_l = " + cnfs(n_l, 5, '0') +
+         " fpf = " + cnfs(fpf, 4, '0') + " ");
+  record("cvf = " + cnfs(cvf, 5, '0', 3) + " laf = " + cnfs(laf, 5, '0', 3) + " ");
+
+  dt = par.get<float>("timestep", DT_DEF);
+  if (!(dt > 0.0f && dt < 1.0f)) throw Error(CCP_ERR_PARAM, "timestep out of range");
+  spf = par.get<uint32_t>("steps_per_frame", (uint32_t)(1.0 / dt));
+  tpb = par.get<uint32_t>("threads_per_block", 128u);
+  if (!(1 <= spf && spf < 10000)) throw Error(CCP_ERR_PARAM, "steps_per_frame out of range");
+  if (!(1 <= tpb && tpb < 1025)) throw Error(CCP_ERR_PARAM, "threads_per_block out of range");
+  record("spf = " + cnfs(spf, 4, '0') + " tpb = " + cnfs(tpb, 4, '0') + " ");
+  sd = (float)std::sqrt(2.0 * K_B * (double)T * (double)dt);
+
+  // ---- build-defined keys
+  seed = par.has("seed") ? par.get<uint64_t>("seed", 0) : (uint64_t)time(nullptr);
+  std::string ffs = par.get<std::string>("force_field", "ccp");
+  if (ffs == "ccp") ff = CCP_FF_CCP;
+  else if (ffs == "lj") ff = CCP_FF_LJ;
+  else throw Error(CCP_ERR_PARAM, "force_field must be ccp or lj");
+  const bool lj = ff == CCP_FF_LJ;
+  const float rc_lj = 2.5f;
+  eps_het = par.get<float>("epsilon_het", 0.5f);
+  eps_lam = par.get<float>("epsilon_lamina", 1.0f);
+  k_b = par.get<float>("bending_constant", lj ? 2.0f : K_BEND);
+  std::string bend = par.get<std::string>("bending", lj ? "all" : "het");
+  if (bend != "all" && bend != "het") throw Error(CCP_ERR_PARAM, "bending must be het or all");
+  bend_all = bend == "all";
+  std::string bond = par.get<std::string>("bond", "harmonic");
+  if (bond != "harmonic" && bond != "fene") throw Error(CCP_ERR_PARAM, "bond must be harmonic or fene");
+  bond_type = bond == "fene";
+  contact_rc = par.get<float>("contact_cutoff", lj ? 1.5f : ACO);
+  float cutoff = lj ? rc_lj : ACO;
+  cutoff = std::max(std::max(cutoff, LCO), contact_rc);
+  csl = par.get<float>("cell_size", lj ? rc_lj : ACO);
+  if (!(csl >= cutoff)) throw Error(CCP_ERR_PARAM, "cell_size below the interaction cutoff");
+  cps = 2 * (uint32_t)std::ceil(d_m / csl);
+  if (cps < 4) cps = 4;
+  if ((uint64_t)cps * cps * cps >= (1ull << 31)) throw Error(CCP_ERR_PARAM, "cell_size too small for the nucleus");
+  n_c = cps * cps * cps;
+  nblk = (n_c + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  cap = par.get<uint32_t>("cell_capacity", 32u);
+  if (!(1 <= cap && cap <= 1024)) throw Error(CCP_ERR_PARAM, "cell_capacity out of range");
+  fcap = par.get<float>("force_cap", lj ? 0.2f / dt : 0.0f);
+  if (!(fcap >= 0.0f)) throw Error(CCP_ERR_PARAM, "force_cap out of range");
+  graph_steps = par.get<uint32_t>("graph_steps", 64u);
+  if (!(1 <= graph_steps && graph_steps <= 4096)) throw Error(CCP_ERR_PARAM, "graph_steps out of range");
+
+  // chains: ccp's chromosome limits when N = 30386 (chrdat.cuh:27-34),
+  // otherwise number_of_chains equal parts or an explicit chain_lengths list
+  chain_off.clear();
+  if (par.has("chain_lengths")) {
+    std::string s = par.get<std::string>("chain_lengths", "");
+    std::replace(s.begin(), s.end(), ',', ' ');
+    std::istringstream is(s);
+    chain_off.push_back(0);
+    long long L;
+    while (is >> L) {
+      if (L <= 0) throw Error(CCP_ERR_PARAM, "chain_lengths out of range");
+      chain_off.push_back(chain_off.back() + (uint32_t)L);
+    }
+    if (chain_off.back() != N || chain_off.size() < 2) throw Error(CCP_ERR_PARAM, "chain_lengths must sum to N");
+  } else {
+    uint32_t nch = par.get<uint32_t>("number_of_chains", 0u);
+    if (nch == 0 && N == N_DEF) {
+      chain_off.assign(CHR_LIM, CHR_LIM + 13);
+    } else {
+      if (nch == 0) nch = 1;
+      if (nch > N) throw Error(CCP_ERR_PARAM, "number_of_chains out of range");
+      chain_off.push_back(0);
+      for (uint32_t c = 0; c < nch; ++c) chain_off.push_back(chain_off.back() + N / nch + (c < N % nch ? 1 : 0));
+    }
+  }
+
+  seq_file = par.get<std::string>("sequence_file", "Program/data/sequence.txt");
+  std::string seq = par.get<std::string>("sequence", lj ? "het_lad" : "ccp");
+  if (seq != "ccp" && seq != "het_lad") throw Error(CCP_ERR_PARAM, "sequence must be ccp or het_lad");
+  seq_mode = seq == "het_lad";
+  het_fraction = par.get<float>("het_fraction", 0.3f);
+  het_mean = par.get<float>("het_block_mean", 20.0f);
+  lad_fraction = par.get<float>("lad_fraction", 0.1f);
+  std::string lp = par.get<std::string>("lad_placement", "ends");
+  if (lp != "ends" && lp != "uniform") throw Error(CCP_ERR_PARAM, "lad_placement must be ends or uniform");
+  lad_ends = lp == "ends";
+  if (!(het_fraction > 0.0f && het_fraction < 1.0f)) throw Error(CCP_ERR_PARAM, "het_fraction out of range");
+  if (!(lad_fraction >= 0.0f && lad_fraction < 1.0f)) throw Error(CCP_ERR_PARAM, "lad_fraction out of range");
+  if (!(het_mean >= 1.0f)) throw Error(CCP_ERR_PARAM, "het_block_mean out of range");
+
+  cbin_size = par.get<uint32_t>("contact_bin_size", std::max<uint32_t>(1u, (N + 1023) / 1024));
+  if (cbin_size < 1) throw Error(CCP_ERR_PARAM, "contact_bin_size out of range");
+  cbins = (N + cbin_size - 1) / cbin_size;
+  if ((uint64_t)cbins * cbins > (1ull << 28)) throw Error(CCP_ERR_PARAM, "contact map too large");
+  cinterval = par.get<uint32_t>("contact_interval", 0u);
+
+  hr.assign(3 * (size_t)N, 0.0f);
+  hlr.assign(3 * (size_t)n_l, 0.0f);
+  hfl.assign(N, 0);
+}
+
+void ccp_sim::fill_kparams() {
+  KParams &k = kp;
+  k.N = (int)N;
+  k.n_l = (int)n_l;
+  k.cps = (int)cps;
+  k.half = (int)(cps / 2);
+  k.n_c = (int)n_c;
+  k.cap = (int)cap;
+  k.nblk = (int)nblk;
+  k.csl = csl;
+  k.R_n = R_n; k.R_o = R_o; k.R_b = R_b;
+  k.noc = noc; k.boc = boc; k.nod = nod; k.bod = bod;
+  const bool lj = ff == CCP_FF_LJ;
+  k.k_e = K_E;
+  k.l_0 = L_0;
+  k.k_b = k_b;
+  k.bend_all = bend_all;
+  k.bond_type = bond_type;
+  k.fene_k = 30.0f;
+  k.fene_r02 = 1.5f * 1.5f;
+  k.e_p = E_P;
+  k.rco = RCO;
+  k.rco2 = RCO * RCO;
+  k.aco2 = ACO * ACO;
+  k.lfac = E_L * (8.0f / (3.0f * LCO * LCO));
+  k.lco2 = LCO * LCO;
+  k.inv_lco2 = 1.0f / (LCO * LCO);
+  k.eps_wca = 1.0f;
+  k.sig2 = 1.0f;
+  k.rm2 = (float)std::pow(2.0, 1.0 / 3.0);
+  k.eps_het = eps_het;
+  k.rc_lj2 = 2.5f * 2.5f;
+  k.eps_wall = 1.0f;
+  k.eps_lam = eps_lam;
+  k.rc_lam2 = 2.5f * 2.5f;
+  (void)lj;
+  k.dt = dt;
+  k.half_dt = 0.5f * dt;
+  k.sd = sd;
+  k.fcap2 = fcap * fcap;
+  k.seed_lo = (uint32_t)seed;
+  k.seed_hi = (uint32_t)(seed >> 32);
+  k.contact_rc2 = contact_rc * contact_rc;
+  k.contact_bin_size = (int)cbin_size;
+  k.contact_bins = (int)cbins;
+}
+
+void ccp_sim::allocate() {
+  hip_check(hipGetDevice(&dev));
+  hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  r.alloc(N); er.alloc(N); pstar.alloc(N); spos.alloc(N); aux.alloc(N); soi.alloc(N);
+  lr.alloc(n_l); lspos.alloc(n_l);
+  cstart.alloc(n_c + 1); lstart.alloc(n_c + 1); count.alloc(n_c);
+  bs0.alloc(nblk); bs1.alloc(nblk);
+  bucket.alloc((size_t)n_c * cap);
+  flags.alloc(FLAG_COUNT);
+  cmap.alloc((size_t)cbins * cbins);
+  dstep.alloc(1);
+  dcount.alloc(1);
+  hip_check(hipMemset(count.p, 0, n_c * sizeof(uint32_t)));
+  hip_check(hipMemset(bs0.p, 0, nblk * sizeof(uint32_t)));
+  hip_check(hipMemset(bs1.p, 0, nblk * sizeof(uint32_t)));
+  hip_check(hipMemset(flags.p, 0, FLAG_COUNT * sizeof(uint32_t)));
+  hip_check(hipMemset(cmap.p, 0, (size_t)cbins * cbins * sizeof(uint32_t)));
+  hip_check(hipMemset(dstep.p, 0, sizeof(uint64_t)));
+  hip_check(hipMemset(cstart.p, 0, (n_c + 1) * sizeof(uint32_t)));
+  hip_check(hipMemset(lstart.p, 0, (n_c + 1) * sizeof(uint32_t)));
+  clean_bs = 0;
+}
+
+uint32_t ccp_sim::wbits(uint32_t i) const {
+  uint32_t w = i;
+  if (hfl[i] & CCP_HET) w |= W_HET;
+  if (hfl[i] & CCP_LAD) w |= W_LAD;
+  auto it = std::upper_bound(chain_off.begin(), chain_off.end(), i);
+  uint32_t s = *(it - 1), e = *it;
+  if (i >= s + 2) w |= W_B0;
+  if (i >= s + 1) w |= W_B1;
+  if (i + 1 <= e - 1) w |= W_B2;
+  if (i + 2 <= e - 1) w |= W_B3;
+  return w;
+}
+
+void ccp_sim::upload_positions() {
+  std::vector<float4> h(N);
+  for (uint32_t i = 0; i < N; ++i) {
+    uint32_t w = wbits(i);
+    float wf;
+    std::memcpy(&wf, &w, 4);
+    h[i] = make_float4(hr[3 * i], hr[3 * i + 1], hr[3 * i + 2], wf);
+  }
+  hip_check(hipMemcpyAsync(r.p, h.data(), N * sizeof(float4), hipMemcpyHostToDevice, stream));
+  hip_check(hipStreamSynchronize(stream));
+  grid_valid = false;
+}
+
+void ccp_sim::upload_lbs() {
+  std::vector<float4> h(std::max<uint32_t>(n_l, 1));
+  for (uint32_t l = 0; l < n_l; ++l) {
+    float wf;
+    std::memcpy(&wf, &l, 4);
+    h[l] = make_float4(hlr[3 * l], hlr[3 * l + 1], hlr[3 * l + 2], wf);
+  }
+  if (n_l) hip_check(hipMemcpyAsync(lr.p, h.data(), n_l * sizeof(float4), hipMemcpyHostToDevice, stream));
+  hip_check(hipStreamSynchronize(stream));
+  lgrid_valid = false;
+}
+
+void ccp_sim::download_positions() {
+  std::vector<float4> h(N);
+  hip_check(hipMemcpyAsync(h.data(), r.p, N * sizeof(float4), hipMemcpyDeviceToHost, stream));
+  hip_check(hipStreamSynchronize(stream));
+  for (uint32_t i = 0; i < N; ++i) {
+    hr[3 * i] = h[i].x;
+    hr[3 * i + 1] = h[i].y;
+    hr[3 * i + 2] = h[i].z;
+  }
+}
+
+void ccp_sim::check_flags() {
+  uint32_t f[FLAG_COUNT];
+  hip_check(hipMemcpyAsync(f, flags.p, sizeof(f), hipMemcpyDeviceToHost, stream));
+  hip_check(hipStreamSynchronize(stream));
+  if (f[FLAG_OVERFLOW] || f[FLAG_OUTSIDE] || f[FLAG_NONFINITE]) {
+    hip_check(hipMemset(flags.p, 0, sizeof(f)));
+    if (f[FLAG_NONFINITE]) throw Error(CCP_ERR_STATE, "non-finite bead position");
+    if (f[FLAG_OUTSIDE]) throw Error(CCP_ERR_STATE, "bead left the cell grid");
+    throw Error(CCP_ERR_STATE, "cell_capacity exceeded; raise cell_capacity");
+  }
+}
+
+// Bin n_obj positions and order them into (out_spos, out_cstart[, out_soi]).
+void ccp_sim::build_grid(const float4 *pos, int n_obj, float4 *out_spos, uint32_t *out_cstart, uint32_t *out_soi) {
+  KParams k = kp;
+  launch_bin(stream, k, n_obj, pos, count.p, bs(clean_bs), bucket.p, flags.p);
+  launch_order(stream, k, n_obj, count.p, bs(clean_bs), bs(clean_bs ^ 1), bucket.p, pos, out_spos, out_soi, out_cstart,
+               flags.p, nullptr);
+  hip_check(hipGetLastError());
+  clean_bs ^= 1;
+}
+
+void ccp_sim::ensure_grids() {
+  if (!lgrid_valid) {
+    build_grid(lr.p, (int)n_l, lspos.p, lstart.p, nullptr);
+    lgrid_valid = true;
+  }
+  if (!grid_valid) {
+    build_grid(r.p, (int)N, spos.p, cstart.p, nullptr);
+    grid_valid = true;
+  }
+}
+
+void ccp_sim::sync_step() {
+  if (dstep_dirty) {
+    hip_check(hipMemcpyAsync(dstep.p, &step, sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    hip_check(hipStreamSynchronize(stream));
+    dstep_dirty = false;
+  }
+}
+
+void ccp_sim::launch_stage(int mode, int stage) {
+  KParams k = mode_params(mode);
+  KPtrs q{};
+  q.spos = spos.p;
+  q.cstart = cstart.p;
+  q.lspos = lspos.p;
+  q.lstart = lstart.p;
+  q.pos = stage == 1 ? r.p : er.p;
+  q.out0 = stage == 1 ? er.p : r.p;
+  q.pstar = pstar.p;
+  q.count = count.p;
+  q.blocksum = bs(clean_bs);
+  q.bucket = bucket.p;
+  q.flags = flags.p;
+  q.step = dstep.p;
+  launch_force(stream, mode, stage, k, q);
+  launch_order(stream, k, (int)N, count.p, bs(clean_bs), bs(clean_bs ^ 1), bucket.p, q.out0, spos.p, nullptr,
+               cstart.p, flags.p, stage == 2 ? dstep.p : nullptr);
+  clean_bs ^= 1;
+}
+
+// n steps without contact sampling: whole graphs of graph_steps, then eager.
+void ccp_sim::run_chunk(uint64_t n, int mode) {
+  if (n == 0) return;
+  const uint64_t G = graph_steps;
+  if (n >= G) {
+    hipGraphExec_t &ge = graphs[mode][clean_bs];
+    if (!ge) {
+      const int cb = clean_bs;
+      hipGraph_t g;
+      hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      for (uint64_t s = 0; s < G; ++s) launch_step(mode);
+      hip_check(hipStreamEndCapture(stream, &g));
+      hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      hip_check(hipGraphDestroy(g));
+      clean_bs = cb; // capture recorded the launches; replays do the work
+    }
+    const int cb0 = clean_bs;
+    while (n >= G) {
+      hip_check(hipGraphLaunch(ge, stream));
+      n -= G;
+    }
+    clean_bs = cb0; // each step flips the clean buffer twice
+  }
+  for (uint64_t s = 0; s < n; ++s) launch_step(mode);
+  hip_check(hipGetLastError());
+}
+
+void ccp_sim::run_steps(uint64_t n, int mode) {
+  ensure_grids();
+  sync_step();
+  while (n > 0) {
+    uint64_t chunk = n;
+    if (cinterval && mode != MODE_ICG) chunk = std::min<uint64_t>(n, cinterval - step % cinterval);
+    run_chunk(chunk, mode);
+    step += chunk;
+    n -= chunk;
+    if (cinterval && mode != MODE_ICG && step % cinterval == 0) {
+      launch_contacts(stream, kp, spos.p, cstart.p, cmap.p);
+      ++n_samples;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- initial condition
+// chrsim::set_lbs_positions (chrsim.cu:596-631): uniform directions on the
+// sphere of radius R_n - rco, rejecting sites closer than 2 lco to another
+// or inside the opening; a hash grid of cell 2 lco replaces the O(n_l^2) scan.
+void ccp_sim::set_lbs_positions(HostRng &g) {
+  if (n_l == 0) return;
+  const float hc = 2.0f * LCO;
+  const int hs = 2 * (int)std::ceil(R_n / hc) + 3;
+  std::map<long long, std::vector<uint32_t>> hash;
+  auto key = [&](V3 p, int dx, int dy, int dz) {
+    long long ix = (long long)std::floor(p.x / hc) + hs / 2 + dx, iy = (long long)std::floor(p.y / hc) + hs / 2 + dy,
+              iz = (long long)std::floor(p.z / hc) + hs / 2 + dz;
+    return (iz * hs + iy) * hs + ix;
+  };
+  uint64_t attempts = 0;
+  for (uint32_t i_l = 0; i_l < n_l; ++i_l) {
+    if (++attempts > 200000000ull) throw Error(CCP_ERR_PARAM, "unable to place lamina binding sites");
+    float ran = g.uniform();
+    float theta = (float)std::acos(1.0 - 2.0 * ran);
+    ran = g.uniform();
+    float phi = (float)(2.0 * M_PI * ran);
+    V3 dir = {std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)};
+    V3 p = (R_n - RCO) * dir;
+    bool ok = true;
+    for (int dz = -1; dz <= 1 && ok; ++dz)
+      for (int dy = -1; dy <= 1 && ok; ++dy)
+        for (int dx = -1; dx <= 1 && ok; ++dx) {
+          auto it = hash.find(key(p, dx, dy, dz));
+          if (it == hash.end()) continue;
+          for (uint32_t j : it->second) {
+            V3 q = {hlr[3 * j], hlr[3 * j + 1], hlr[3 * j + 2]};
+            if (length(q - p) < 2.0f * LCO) { ok = false; break; }
+          }
+        }
+    if (p.z / length(p) > noc) ok = false;
+    if (!ok) { --i_l; continue; }
+    hlr[3 * i_l] = p.x; hlr[3 * i_l + 1] = p.y; hlr[3 * i_l + 2] = p.z;
+    hash[key(p, 0, 0, 0)].push_back(i_l);
+  }
+}
+
+// chrsim::set_particle_types (chrsim.cu:634-675) in ccp mode; het/LAD
+// generator otherwise (het blocks Geometric(het_block_mean) at het_fraction,
+// LAD blocks placed with an end-weighted profile).
+void ccp_sim::set_particle_types(HostRng &g) {
+  if (seq_mode == 0) {
+    if (N == N_DEF) {
+      std::ifstream f(seq_file);
+      if (!f.is_open()) throw Error(CCP_ERR_IO, "unable to open " + seq_file);
+      char c;
+      for (uint32_t i = 0; i < N; ++i) {
+        f >> c;
+        hfl[i] = (c == 'A') ? (CCP_HET | CCP_LAD) : 0;
+      }
+      if (f.fail()) throw Error(CCP_ERR_IO, "failed to read sequence");
+    } else {
+      uint32_t cde = 0;
+      bool a = false; // current type LNDe
+      for (uint32_t i = 0; i < N; ++i) {
+        if (i == cde) {
+          float edl;
+          do {
+            float ran = g.uniform();
+            edl = -MDL * std::log(ran);
+          } while ((edl / MDL) > 5 || edl < 1.0f);
+          cde = (uint32_t)(i + edl);
+          a = !a;
+        }
+        hfl[i] = a ? (CCP_HET | CCP_LAD) : 0;
+      }
+    }
+    return;
+  }
+  auto geometric = [&](double mean) {
+    double p = 1.0 / mean;
+    if (p >= 1.0) return (uint32_t)1;
+    return 1u + (uint32_t)std::floor(std::log((double)g.uniform()) / std::log(1.0 - p));
+  };
+  const double f = het_fraction, m_h = het_mean, m_e = het_mean * (1.0 - f) / f;
+  for (size_t c = 0; c + 1 < chain_off.size(); ++c) {
+    const uint32_t s = chain_off[c], e = chain_off[c + 1], L = e - s;
+    bool het = g.uniform() < (float)f;
+    uint32_t i = s;
+    while (i < e) {
+      uint32_t len = geometric(het ? m_h : m_e);
+      for (uint32_t k = 0; k < len && i < e; ++k, ++i) hfl[i] = het ? CCP_HET : 0;
+      het = !het;
+    }
+    // LAD blocks
+    const uint32_t target = (uint32_t)std::lround(lad_fraction * L);
+    const double lam = std::max(1.0, lad_fraction * L / 2.0);
+    uint32_t marked = 0;
+    uint64_t guard = 0;
+    while (marked < target && ++guard < 100ull * L + 1000) {
+      uint32_t pos;
+      if (lad_ends) {
+        double d = -lam * std::log((double)g.uniform());
+        if (d >= L) continue;
+        uint32_t off = (uint32_t)d;
+        pos = g.uniform() < 0.5f ? s + off : e - 1 - off;
+      } else {
+        pos = s + std::min<uint32_t>(L - 1, (uint32_t)(g.uniform() * L));
+      }
+      uint32_t len = geometric(m_h);
+      for (uint32_t k = 0; k < len && marked < target; ++k) {
+        uint32_t q = pos + k;
+        if (q >= e) break;
+        if (!(hfl[q] & CCP_LAD)) {
+          hfl[q] |= CCP_LAD;
+          ++marked;
+        }
+      }
+    }
+  }
+}
+
+// chrsim::perform_random_walk (chrsim.cu:719-802): confined random walk with
+// bending-distributed bond angles, Gaussian bond lengths and ccp's
+// backtracking rule.
+void ccp_sim::perform_random_walk(HostRng &g, uint32_t i_s, uint32_t i_e, V3 dir, float mda) {
+  const float len_d = length(dir);
+  const bool cone = mda < (float)M_PI && len_d > 0.0f;
+  const double iT = 1.0 / ((double)K_B * T);
+  const double bend_scale = (ff == CCP_FF_LJ) ? 0.5 * k_b : 0.5; // ccp: 0.5 iT for k_b = 1
+  const double ke = K_E;
+  auto rand_dir = [&]() {
+    float ran = g.uniform();
+    float theta = (float)std::acos(1.0 - 2.0 * ran);
+    ran = g.uniform();
+    float phi = (float)(2.0 * M_PI * ran);
+    return V3{std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)};
+  };
+  auto in_cone = [&](V3 p, float d_r) { return !cone || !(std::acos(dot(p, dir) / (d_r * len_d)) > mda); };
+  V3 old_dir;
+  uint64_t guard = 0;
+  for (;;) {
+    V3 rd = rand_dir();
+    float ran = g.uniform();
+    float d_r = (float)((R_n - MIS) * std::pow((double)ran, 1.0 / 3));
+    V3 p = d_r * rd;
+    hr[3 * i_s] = p.x; hr[3 * i_s + 1] = p.y; hr[3 * i_s + 2] = p.z;
+    old_dir = rd;
+    if (in_cone(p, d_r)) break;
+    if (++guard > 100000000ull) throw Error(CCP_ERR_PARAM, "unable to start random walk");
+  }
+  uint32_t att = 0;
+  guard = 0;
+  for (uint32_t i_p = i_s + 1; i_p < i_e; ++i_p) {
+    if (++guard > 4000000000ull) throw Error(CCP_ERR_PARAM, "unable to complete random walk");
+    V3 rd = rand_dir();
+    V3 per = cross(old_dir, rd);
+    per = (1.0f / std::sqrt(dot(per, per))) * per;
+    float ran = g.uniform();
+    const double beta = bend_scale * iT;
+    float angle_b = (float)std::acos(1.0 + std::log(1.0 - (1.0 - std::exp(-2.0 * beta)) * ran) / beta);
+    V3 nd = std::cos(angle_b) * old_dir + std::sin(angle_b) * per;
+    ran = g.uniform();
+    float len_b = (float)(L_0 + std::sqrt(2.0 / (ke * iT)) * erfinv(2.0 * ran - 1.0));
+    V3 prev = {hr[3 * (i_p - 1)], hr[3 * (i_p - 1) + 1], hr[3 * (i_p - 1) + 2]};
+    V3 p = len_b * nd + prev;
+    hr[3 * i_p] = p.x; hr[3 * i_p + 1] = p.y; hr[3 * i_p + 2] = p.z;
+    bool ok = std::isfinite(p.x) && std::isfinite(p.y) && std::isfinite(p.z);
+    float d_r = length(p);
+    if ((R_n - d_r) < MIS) ok = false;
+    if (!in_cone(p, d_r)) ok = false;
+    if (ok) {
+      att = 0;
+      old_dir = nd;
+    } else {
+      ++att;
+      if (att > 1024) i_p = i_s + 1 + (i_p - i_s) * 3 / 4;
+      else --i_p;
+    }
+  }
+}
+
+// chrsim::set_particle_positons (chrsim.cu:678-716): one walk per chain;
+// several chains (ccp: the 12 chromosomes of N = 30386) start inside cones
+// around shuffled icosahedron directions.
+void ccp_sim::set_particle_positions(HostRng &g) {
+  const size_t nch = chain_off.size() - 1;
+  if (nch > 1) {
+    const float phi = (float)((1.0 + std::sqrt(5.0)) / 2.0);
+    const float mda = (float)(M_PI / 2.0 - std::atan(phi));
+    V3 dirs[12] = {{0, +1, +phi}, {+1, +phi, 0}, {+phi, 0, +1}, {0, +1, -phi}, {+1, -phi, 0}, {-phi, 0, +1},
+                   {0, -1, +phi}, {-1, +phi, 0}, {+phi, 0, -1}, {0, -1, -phi}, {-1, -phi, 0}, {-phi, 0, -1}};
+    for (uint32_t i_d = 11; i_d > 0; --i_d) {
+      float ran = g.uniform();
+      uint32_t i_r = (uint32_t)((i_d + 1) * (1.0 - ran));
+      std::swap(dirs[i_d], dirs[i_r]);
+    }
+    for (size_t c = 0; c < nch; ++c) perform_random_walk(g, chain_off[c], chain_off[c + 1], dirs[c % 12], mda);
+  } else {
+    perform_random_walk(g, 0, N, {0, 0, 0}, (float)M_PI);
+  }
+}
+
+uint64_t ccp_sim::count_overlaps(float dmin) {
+  ensure_grids();
+  hip_check(hipMemsetAsync(dcount.p, 0, sizeof(unsigned long long), stream));
+  launch_overlaps(stream, kp, spos.p, cstart.p, dmin * dmin, dcount.p);
+  unsigned long long c = 0;
+  hip_check(hipMemcpyAsync(&c, dcount.p, sizeof(c), hipMemcpyDeviceToHost, stream));
+  hip_check(hipStreamSynchronize(stream));
+  return c;
+}
+
+// chrsim::generate_initial_condition (chrsim.cu:470-522)
+void ccp_sim::generate_initial_condition() {
+  HostRng g(seed);
+  set_lbs_positions(g);
+  set_particle_types(g);
+  set_particle_positions(g);
+  upload_lbs();
+  upload_positions();
+  uint64_t po = count_overlaps(MIS), ipo = po;
+  uint32_t rounds = 0;
+  while (po > 0) {
+    progress((float)(100.0 * (ipo - po) / ipo));
+    if (++rounds > 10000) throw Error(CCP_ERR_STATE, "unable to remove bead overlaps");
+    run_steps(spf, MODE_ICG);
+    check_flags();
+    po = count_overlaps(MIS);
+  }
+  download_positions();
+  step = 0;
+  dstep_dirty = true;
+  i_f = 0;
+  t = 0.0f;
+  record("initial condition generated");
+}
+
+void ccp_sim::write_frame_bin(std::ofstream &f) {
+  // minimal .trr frame (chrdat.cu:168-201)
+  uint32_t tb;
+  std::memcpy(&tb, &t, 4);
+  uint32_t header[18] = {1993, 1, 0, 0, 0, 0, 0, 0, 0, 0, 3 * N * 4, 0, 0, N, i_f, 0, tb, 0};
+  f.write(reinterpret_cast<const char *>(header), sizeof(header));
+  f.write(reinterpret_cast<const char *>(hr.data()), 12 * (size_t)N);
+  if (f.fail()) throw Error(CCP_ERR_IO, "failed to write frame to binary file");
+}
+
+// ---------------------------------------------------------------- C ABI
+namespace {
+thread_local std::string g_create_error;
+
+template <class F> int guarded(ccp_sim *h, F &&fn) {
+  if (!h) {
+    g_create_error = "null handle";
+    return CCP_ERR_ARG;
+  }
+  try {
+    hip_check(hipSetDevice(h->dev));
+    fn();
+    return CCP_OK;
+  } catch (const Error &e) {
+    h->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    h->err = e.what();
+    return CCP_ERR_ARG;
+  }
+}
+
+void need(bool ok, const char *what) {
+  if (!ok) throw Error(CCP_ERR_ARG, what);
+}
+
+const char *CKPT_MAGIC = "CCPMI001";
+} // namespace
+
+extern "C" {
+
+ccp_sim *ccp_create(const char *param_text) { return ccp_create_logged(param_text, nullptr, nullptr); }
+
+ccp_sim *ccp_create_logged(const char *param_text, ccp_log_fn fn, void *user) {
+  std::unique_ptr<ccp_sim> h(new ccp_sim());
+  h->log_fn = fn;
+  h->log_user = user;
+  try {
+    ParMap par(param_text);
+    h->configure(par);
+    h->fill_kparams();
+    h->allocate();
+    h->upload_positions();
+    h->upload_lbs();
+    return h.release();
+  } catch (const std::exception &e) {
+    g_create_error = e.what();
+    return nullptr;
+  }
+}
+
+void ccp_destroy(ccp_sim *h) { delete h; }
+
+const char *ccp_last_error(const ccp_sim *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int ccp_set_logger(ccp_sim *h, ccp_log_fn fn, void *user) {
+  return guarded(h, [&] {
+    h->log_fn = fn;
+    h->log_user = user;
+  });
+}
+
+int ccp_get_info(const ccp_sim *h, ccp_info *o) {
+  if (!h || !o) return CCP_ERR_ARG;
+  o->N = h->N;
+  o->n_l = h->n_l;
+  o->n_chains = (uint32_t)h->chain_off.size() - 1;
+  o->force_field = (uint32_t)h->ff;
+  o->cps = h->cps;
+  o->n_c = h->n_c;
+  o->cell_capacity = h->cap;
+  o->spf = h->spf;
+  o->fpf = h->fpf;
+  o->contact_bins = h->cbins;
+  o->contact_bin_size = h->cbin_size;
+  o->contact_interval = h->cinterval;
+  o->csl = h->csl;
+  o->R_n = h->R_n;
+  o->R_o = h->R_o;
+  o->R_b = h->R_b;
+  o->T = h->T;
+  o->dt = h->dt;
+  o->sd = h->sd;
+  o->contact_cutoff = h->contact_rc;
+  o->force_cap = h->fcap;
+  o->seed = h->seed;
+  return CCP_OK;
+}
+
+int ccp_generate_initial_condition(ccp_sim *h) {
+  return guarded(h, [&] { h->generate_initial_condition(); });
+}
+
+int ccp_save_checkpoint(ccp_sim *h, const char *path) {
+  return guarded(h, [&] {
+    need(path != nullptr, "null path");
+    std::ofstream f(path, std::ios::binary);
+    if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + path);
+    h->download_positions();
+    f.write(reinterpret_cast<const char *>(&h->i_f), 4);
+    f.write(reinterpret_cast<const char *>(&h->t), 4);
+    for (uint32_t i = 0; i < h->N; ++i) {
+      int32_t pt = (h->hfl[i] & CCP_HET) ? 0 : 1;
+      f.write(reinterpret_cast<const char *>(&pt), 4);
+    }
+    f.write(reinterpret_cast<const char *>(h->hr.data()), 12 * (size_t)h->N);
+    f.write(reinterpret_cast<const char *>(h->hlr.data()), 12 * (size_t)h->n_l);
+    f.write(CKPT_MAGIC, 8);
+    f.write(reinterpret_cast<const char *>(&h->seed), 8);
+    f.write(reinterpret_cast<const char *>(&h->step), 8);
+    f.write(reinterpret_cast<const char *>(&h->N), 4);
+    f.write(reinterpret_cast<const char *>(h->hfl.data()), h->N);
+    if (f.fail()) throw Error(CCP_ERR_IO, "failed to write checkpoint");
+    h->record("simulation checkpoint saved");
+  });
+}
+
+int ccp_load_checkpoint(ccp_sim *h, const char *path) {
+  return guarded(h, [&] {
+    need(path != nullptr, "null path");
+    std::ifstream f(path, std::ios::binary);
+    if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + path);
+    uint32_t i_f;
+    float t;
+    std::vector<int32_t> pt(h->N);
+    std::vector<float> r(3 * (size_t)h->N), lr(3 * (size_t)h->n_l);
+    f.read(reinterpret_cast<char *>(&i_f), 4);
+    f.read(reinterpret_cast<char *>(&t), 4);
+    f.read(reinterpret_cast<char *>(pt.data()), 4 * (size_t)h->N);
+    f.read(reinterpret_cast<char *>(r.data()), 12 * (size_t)h->N);
+    f.read(reinterpret_cast<char *>(lr.data()), 12 * (size_t)h->n_l);
+    if (f.fail()) throw Error(CCP_ERR_IO, "failed to read checkpoint");
+    h->i_f = i_f;
+    h->t = t;
+    h->hr = r;
+    h->hlr = lr;
+    for (uint32_t i = 0; i < h->N; ++i) h->hfl[i] = pt[i] == 0 ? (CCP_HET | CCP_LAD) : 0;
+    h->step = (uint64_t)i_f * h->spf;
+    char magic[8];
+    f.read(magic, 8);
+    if (f && std::memcmp(magic, CKPT_MAGIC, 8) == 0) {
+      uint64_t seed, step;
+      uint32_t n;
+      f.read(reinterpret_cast<char *>(&seed), 8);
+      f.read(reinterpret_cast<char *>(&step), 8);
+      f.read(reinterpret_cast<char *>(&n), 4);
+      if (f && n == h->N) {
+        std::vector<uint8_t> fl(n);
+        f.read(reinterpret_cast<char *>(fl.data()), n);
+        if (f) {
+          h->hfl = fl;
+          h->seed = seed;
+          h->step = step;
+          h->fill_kparams();
+        }
+      }
+    }
+    h->dstep_dirty = true;
+    h->upload_positions();
+    h->upload_lbs();
+    h->record("simulation checkpoint loaded");
+  });
+}
+
+int ccp_write_frame_txt(ccp_sim *h, const char *path) {
+  return guarded(h, [&] {
+    need(path != nullptr, "null path");
+    std::ofstream f(path);
+    if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + path);
+    h->download_positions();
+    // chrdat::write_frame_txt (chrdat.cu:107-133)
+    f << h->i_f << " " << h->t << "\n";
+    f << cnfs(h->N, 5, ' ') << "\n";
+    for (uint32_t i = 0; i < h->N; ++i) {
+      char ptc = (h->hfl[i] & CCP_HET) ? 'A' : 'B';
+      f << std::setw(5) << i + 1 << std::left << std::setw(5) << ptc;
+      f << std::right << std::setw(5) << ptc << std::setw(5) << i + 1;
+      f << cnfs(h->hr[3 * i], 8, ' ', 3) << cnfs(h->hr[3 * i + 1], 8, ' ', 3) << cnfs(h->hr[3 * i + 2], 8, ' ', 3);
+      f << "\n";
+    }
+    f << cnfs(0.0, 10, ' ', 5) << cnfs(0.0, 10, ' ', 5) << cnfs(0.0, 10, ' ', 5) << "\n";
+    if (f.fail()) throw Error(CCP_ERR_IO, "failed to write frame to text file");
+  });
+}
+
+int ccp_read_frame_txt(ccp_sim *h, const char *path) {
+  return guarded(h, [&] {
+    need(path != nullptr, "null path");
+    std::ifstream f(path);
+    if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + path);
+    // chrdat::read_frame_txt (chrdat.cu:136-165)
+    std::string aux;
+    char ptc;
+    uint32_t i_f;
+    float t;
+    f >> i_f >> t >> aux;
+    std::vector<float> r(3 * (size_t)h->N);
+    std::vector<uint8_t> fl(h->N);
+    for (uint32_t i = 0; i < h->N; ++i) {
+      f >> aux >> ptc >> aux;
+      fl[i] = (ptc == 'A') ? (CCP_HET | CCP_LAD) : 0;
+      f >> r[3 * i] >> r[3 * i + 1] >> r[3 * i + 2];
+    }
+    f >> aux >> aux >> aux;
+    if (f.fail()) throw Error(CCP_ERR_IO, "failed to read frame from text file");
+    // keep the LAD bits of a het/LAD sequence when the het pattern is unchanged
+    for (uint32_t i = 0; i < h->N; ++i)
+      if (h->ff == CCP_FF_LJ && ((h->hfl[i] & CCP_HET) != 0) == ((fl[i] & CCP_HET) != 0)) fl[i] = h->hfl[i];
+    h->i_f = i_f;
+    h->t = t;
+    h->hr = r;
+    h->hfl = fl;
+    h->upload_positions();
+  });
+}
+
+int ccp_write_lbs_txt(ccp_sim *h, const char *path) {
+  return guarded(h, [&] {
+    need(path != nullptr, "null path");
+    std::ofstream f(path);
+    if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + path);
+    // chrdat::write_lbs_txt (chrdat.cu:229-254)
+    f << "\n";
+    f << cnfs(h->n_l, 5, ' ') << "\n";
+    const char ptc = 'C';
+    for (uint32_t l = 0; l < h->n_l; ++l) {
+      f << std::setw(5) << l + 1 << std::left << std::setw(5) << ptc;
+      f << std::right << std::setw(5) << ptc << std::setw(5) << l + 1;
+      f << cnfs(h->hlr[3 * l], 8, ' ', 3) << cnfs(h->hlr[3 * l + 1], 8, ' ', 3) << cnfs(h->hlr[3 * l + 2], 8, ' ', 3);
+      f << "\n";
+    }
+    f << cnfs(0.0, 10, ' ', 5) << cnfs(0.0, 10, ' ', 5) << cnfs(0.0, 10, ' ', 5) << "\n";
+    if (f.fail()) throw Error(CCP_ERR_IO, "failed to write lamina binding sites to text file");
+  });
+}
+
+int ccp_read_lbs_txt(ccp_sim *h, const char *path) {
+  return guarded(h, [&] {
+    need(path != nullptr, "null path");
+    std::ifstream f(path);
+    if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + path);
+    // The file starts with an empty title line, then the site count
+    // (chrdat.cu:229-236); ccp's reader (chrdat.cu:257-284) is never called
+    // and skips one token too many, so the layout is read as written.
+    std::string line, aux;
+    char ptc;
+    std::getline(f, line);
+    f >> aux;
+    std::vector<float> lr(3 * (size_t)h->n_l);
+    for (uint32_t l = 0; l < h->n_l; ++l) {
+      f >> aux >> ptc >> aux;
+      f >> lr[3 * l] >> lr[3 * l + 1] >> lr[3 * l + 2];
+    }
+    f >> aux >> aux >> aux;
+    if (f.fail()) throw Error(CCP_ERR_IO, "failed to read lamina binding sites from text file");
+    h->hlr = lr;
+    h->upload_lbs();
+  });
+}
+
+int ccp_run_simulation(ccp_sim *h, const char *trr_path) {
+  return guarded(h, [&] {
+    std::ofstream f;
+    if (trr_path) {
+      f.open(trr_path, std::ios::binary);
+      if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + trr_path);
+    }
+    // chrsim::run_simulation (chrsim.cu:558-593)
+    h->lgrid_valid = false;
+    for (uint32_t ffi = 0; ffi < h->fpf; ++ffi) {
+      h->progress((float)(100.0 * ffi / h->fpf));
+      h->run_steps(h->spf, h->ff == CCP_FF_LJ ? MODE_LJ : MODE_CCP);
+      h->download_positions();
+      h->check_flags();
+      ++h->i_f;
+      h->t += h->spf * h->dt;
+      if (trr_path) h->write_frame_bin(f);
+    }
+    h->record("simulation ended");
+  });
+}
+
+int ccp_run_steps(ccp_sim *h, uint64_t n) {
+  return guarded(h, [&] { h->run_steps(n, h->ff == CCP_FF_LJ ? MODE_LJ : MODE_CCP); });
+}
+
+int ccp_synchronize(ccp_sim *h) {
+  return guarded(h, [&] {
+    hip_check(hipStreamSynchronize(h->stream));
+    h->check_flags();
+  });
+}
+
+int ccp_get_time(const ccp_sim *h, uint32_t *i_f, float *t, uint64_t *step) {
+  if (!h) return CCP_ERR_ARG;
+  if (i_f) *i_f = h->i_f;
+  if (t) *t = h->t;
+  if (step) *step = h->step;
+  return CCP_OK;
+}
+
+int ccp_set_time(ccp_sim *h, uint32_t i_f, float t, uint64_t step) {
+  return guarded(h, [&] {
+    h->i_f = i_f;
+    h->t = t;
+    h->step = step;
+    h->dstep_dirty = true;
+  });
+}
+
+int ccp_get_positions(ccp_sim *h, float *xyz, size_t n) {
+  return guarded(h, [&] {
+    need(xyz && n == h->N, "positions size mismatch");
+    h->download_positions();
+    std::memcpy(xyz, h->hr.data(), 12 * (size_t)h->N);
+  });
+}
+
+int ccp_set_positions(ccp_sim *h, const float *xyz, size_t n) {
+  return guarded(h, [&] {
+    need(xyz && n == h->N, "positions size mismatch");
+    std::memcpy(h->hr.data(), xyz, 12 * (size_t)h->N);
+    h->upload_positions();
+  });
+}
+
+int ccp_get_types(ccp_sim *h, int32_t *pt, size_t n) {
+  return guarded(h, [&] {
+    need(pt && n == h->N, "types size mismatch");
+    for (uint32_t i = 0; i < h->N; ++i) pt[i] = (h->hfl[i] & CCP_HET) ? 0 : 1;
+  });
+}
+
+int ccp_set_types(ccp_sim *h, const int32_t *pt, size_t n) {
+  return guarded(h, [&] {
+    need(pt && n == h->N, "types size mismatch");
+    h->download_positions();
+    for (uint32_t i = 0; i < h->N; ++i) h->hfl[i] = pt[i] == 0 ? (CCP_HET | CCP_LAD) : 0;
+    h->upload_positions();
+  });
+}
+
+int ccp_get_flags(ccp_sim *h, uint8_t *fl, size_t n) {
+  return guarded(h, [&] {
+    need(fl && n == h->N, "flags size mismatch");
+    std::memcpy(fl, h->hfl.data(), h->N);
+  });
+}
+
+int ccp_set_flags(ccp_sim *h, const uint8_t *fl, size_t n) {
+  return guarded(h, [&] {
+    need(fl && n == h->N, "flags size mismatch");
+    h->download_positions();
+    for (uint32_t i = 0; i < h->N; ++i) h->hfl[i] = fl[i] & (CCP_HET | CCP_LAD);
+    h->upload_positions();
+  });
+}
+
+int ccp_get_lbs(ccp_sim *h, float *xyz, size_t n_l) {
+  return guarded(h, [&] {
+    need(n_l == h->n_l && (xyz || n_l == 0), "lbs size mismatch");
+    if (n_l) std::memcpy(xyz, h->hlr.data(), 12 * (size_t)h->n_l);
+  });
+}
+
+int ccp_set_lbs(ccp_sim *h, const float *xyz, size_t n_l) {
+  return guarded(h, [&] {
+    need(n_l == h->n_l && (xyz || n_l == 0), "lbs size mismatch");
+    if (n_l) std::memcpy(h->hlr.data(), xyz, 12 * (size_t)h->n_l);
+    h->upload_lbs();
+  });
+}
+
+int ccp_get_chains(ccp_sim *h, uint32_t *off, size_t n) {
+  return guarded(h, [&] {
+    need(off && n == h->chain_off.size(), "chains size mismatch");
+    std::memcpy(off, h->chain_off.data(), 4 * n);
+  });
+}
+
+int ccp_compute_forces(ccp_sim *h, int ff, float *f_xyz, size_t n) {
+  return guarded(h, [&] {
+    need(f_xyz && n == h->N, "forces size mismatch");
+    need(ff == CCP_FF_CCP || ff == CCP_FF_ICG || ff == CCP_FF_LJ, "unknown force field");
+    h->ensure_grids();
+    KParams k = h->mode_params(ff);
+    KPtrs q{};
+    q.spos = h->spos.p;
+    q.cstart = h->cstart.p;
+    q.lspos = h->lspos.p;
+    q.lstart = h->lstart.p;
+    q.pos = h->r.p;
+    q.out0 = h->aux.p;
+    q.flags = h->flags.p;
+    q.step = h->dstep.p;
+    launch_force(h->stream, ff, 0, k, q);
+    hip_check(hipGetLastError());
+    std::vector<float4> out(h->N);
+    hip_check(hipMemcpyAsync(out.data(), h->aux.p, h->N * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    for (uint32_t i = 0; i < h->N; ++i) {
+      f_xyz[3 * i] = out[i].x;
+      f_xyz[3 * i + 1] = out[i].y;
+      f_xyz[3 * i + 2] = out[i].z;
+    }
+  });
+}
+
+int ccp_cell_list(ccp_sim *h, uint32_t *sci, uint32_t *soi, uint32_t *beg, uint32_t *end, size_t n, size_t n_c) {
+  return guarded(h, [&] {
+    need(n == h->N && n_c == h->n_c && sci && soi && beg && end, "cell list size mismatch");
+    h->build_grid(h->r.p, (int)h->N, h->spos.p, h->cstart.p, h->soi.p);
+    h->grid_valid = true;
+    std::vector<uint32_t> cs(h->n_c + 1);
+    hip_check(hipMemcpyAsync(soi, h->soi.p, 4 * n, hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipMemcpyAsync(cs.data(), h->cstart.p, 4 * (n_c + 1), hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    h->check_flags();
+    // ccp layout: beg/end of non-empty cells, 0/0 for empty ones (sugrid.cu:38-77)
+    for (uint32_t c = 0; c < h->n_c; ++c) {
+      const bool full = cs[c + 1] > cs[c];
+      beg[c] = full ? cs[c] : 0u;
+      end[c] = full ? cs[c + 1] : 0u;
+      for (uint32_t s = cs[c]; s < cs[c + 1]; ++s) sci[s] = c;
+    }
+  });
+}
+
+int ccp_noise(ccp_sim *h, uint64_t step, float *z_xyz, uint32_t *attempts, size_t n) {
+  return guarded(h, [&] {
+    need(z_xyz && n == h->N, "noise size mismatch");
+    launch_noise(h->stream, h->kp, step, h->aux.p);
+    hip_check(hipGetLastError());
+    std::vector<float4> out(h->N);
+    hip_check(hipMemcpyAsync(out.data(), h->aux.p, h->N * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    for (uint32_t i = 0; i < h->N; ++i) {
+      z_xyz[3 * i] = out[i].x;
+      z_xyz[3 * i + 1] = out[i].y;
+      z_xyz[3 * i + 2] = out[i].z;
+      if (attempts) std::memcpy(&attempts[i], &out[i].w, 4);
+    }
+  });
+}
+
+int ccp_count_overlaps(ccp_sim *h, float dmin, uint64_t *count) {
+  return guarded(h, [&] {
+    need(count != nullptr, "null count");
+    need(dmin > 0.0f && dmin <= h->csl, "dmin must be in (0, cell_size]");
+    *count = h->count_overlaps(dmin);
+  });
+}
+
+int ccp_contact_sample(ccp_sim *h) {
+  return guarded(h, [&] {
+    h->ensure_grids();
+    launch_contacts(h->stream, h->kp, h->spos.p, h->cstart.p, h->cmap.p);
+    hip_check(hipGetLastError());
+    ++h->n_samples;
+  });
+}
+
+int ccp_contact_reset(ccp_sim *h) {
+  return guarded(h, [&] {
+    hip_check(hipMemsetAsync(h->cmap.p, 0, (size_t)h->cbins * h->cbins * 4, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    h->n_samples = 0;
+  });
+}
+
+int ccp_contact_map(ccp_sim *h, uint32_t *map, size_t n_bins2, uint32_t *n_samples) {
+  return guarded(h, [&] {
+    need(map && n_bins2 == (size_t)h->cbins * h->cbins, "contact map size mismatch");
+    hip_check(hipMemcpyAsync(map, h->cmap.p, 4 * n_bins2, hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    if (n_samples) *n_samples = h->n_samples;
+  });
+}
+
+int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]) {
+  return guarded(h, [&] {
+    need(out && n_steps > 0, "bad timing request");
+    const int mode = h->ff == CCP_FF_LJ ? MODE_LJ : MODE_CCP;
+    h->ensure_grids();
+    h->sync_step();
+    hipEvent_t ev[5];
+    for (auto &e : ev) hip_check(hipEventCreate(&e));
+    double acc[4] = {0, 0, 0, 0};
+    for (uint32_t s = 0; s < n_steps; ++s) {
+      // launch_stage = force launch + order launch; time them separately
+      for (int stage = 1; stage <= 2; ++stage) {
+        KParams k = h->mode_params(mode);
+        KPtrs q{};
+        q.spos = h->spos.p;
+        q.cstart = h->cstart.p;
+        q.lspos = h->lspos.p;
+        q.lstart = h->lstart.p;
+        q.pos = stage == 1 ? h->r.p : h->er.p;
+        q.out0 = stage == 1 ? h->er.p : h->r.p;
+        q.pstar = h->pstar.p;
+        q.count = h->count.p;
+        q.blocksum = h->bs(h->clean_bs);
+        q.bucket = h->bucket.p;
+        q.flags = h->flags.p;
+        q.step = h->dstep.p;
+        hip_check(hipEventRecord(ev[0], h->stream));
+        launch_force(h->stream, mode, stage, k, q);
+        hip_check(hipEventRecord(ev[1], h->stream));
+        launch_order(h->stream, k, (int)h->N, h->count.p, h->bs(h->clean_bs), h->bs(h->clean_bs ^ 1), h->bucket.p,
+                     q.out0, h->spos.p, nullptr, h->cstart.p, h->flags.p, stage == 2 ? h->dstep.p : nullptr);
+        hip_check(hipEventRecord(ev[2], h->stream));
+        h->clean_bs ^= 1;
+        hip_check(hipEventSynchronize(ev[2]));
+        float a, b;
+        hip_check(hipEventElapsedTime(&a, ev[0], ev[1]));
+        hip_check(hipEventElapsedTime(&b, ev[1], ev[2]));
+        acc[(stage - 1) * 2] += a;
+        acc[(stage - 1) * 2 + 1] += b;
+      }
+    }
+    h->step += n_steps;
+    // whole steps back to back (no events between kernels)
+    hip_check(hipEventRecord(ev[3], h->stream));
+    for (uint32_t s = 0; s < n_steps; ++s) h->launch_step(mode);
+    hip_check(hipEventRecord(ev[4], h->stream));
+    hip_check(hipEventSynchronize(ev[4]));
+    float whole;
+    hip_check(hipEventElapsedTime(&whole, ev[3], ev[4]));
+    h->step += n_steps;
+    for (int k = 0; k < 4; ++k) out[k] = (float)(acc[k] / n_steps);
+    out[4] = whole / n_steps;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    h->check_flags();
+  });
+}
+
+} // extern "C"

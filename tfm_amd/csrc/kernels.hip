@@ -1,0 +1,575 @@
+// gfx950 kernels of the ccp Brownian-dynamics step.
+//
+// One stochastic-Heun stage is two launches:
+//   k_force<MODE, STAGE>  one thread per bead in cell-sorted order: pair
+//                         forces from the 9 contiguous neighbour rows of the
+//                         sorted grid, LBS binding, bonds/bending from the
+//                         bead-ordered array, wall force, the Heun update,
+//                         and binning of the updated position into the next
+//                         cell list (wave-aggregated atomics on per-cell
+//                         counters and per-scan-block totals, bucket slot).
+//   k_order               one thread per cell: exclusive scan of the counts
+//                         (block prefix from the per-block totals), then a
+//                         rank-by-bead-index ordering of the cell's bucket, so
+//                         the sorted arrays equal a stable sort by cell (the
+//                         result of ccp's cub::DeviceRadixSort::SortPairs over
+//                         identity values, sugrid.cu:125-133) bit for bit, and
+//                         a gather of the positions into sorted order.
+// Reference behaviour followed: chrsim.cu:13-416, sugrid.cu:15-133.
+
+#include "kernels.hpp"
+
+namespace ccpmi {
+namespace {
+
+__device__ __forceinline__ uint32_t wbits(float4 v) { return __float_as_uint(v.w); }
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Salmon, Moraes, Dror, Shaw (SC'11): counter-based, one call per bead and
+// step; no state array (ccp keeps a curandStatePhilox4_32_10 per bead).
+__device__ __forceinline__ void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
+                                         uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// ((x >> 8) + 1) * 2^-24: uniform in (0, 1], exact in FP32
+__device__ __forceinline__ float u24(uint32_t x) { return (float)((x >> 8) + 1u) * 5.9604644775390625e-8f; }
+
+// Normal triple (Box–Muller) with ccp's redraw of all three while any
+// |z| > 5 (begin_iter, chrsim.cu:348-356). Counter {bead, step lo, step hi, attempt}.
+__device__ __forceinline__ uint32_t noise3(uint32_t k0, uint32_t k1, uint32_t bead, uint64_t step, float &z0,
+                                           float &z1, float &z2) {
+  uint32_t a = 0;
+  for (;; ++a) {
+    uint32_t c0 = bead, c1 = (uint32_t)step, c2 = (uint32_t)(step >> 32), c3 = a;
+    philox10(c0, c1, c2, c3, k0, k1);
+    const float r01 = sqrtf(-2.0f * logf(u24(c0)));
+    float s1, co1;
+    sincospif(2.0f * u24(c1), &s1, &co1);
+    const float r23 = sqrtf(-2.0f * logf(u24(c2)));
+    const float co3 = cospif(2.0f * u24(c3));
+    z0 = r01 * co1;
+    z1 = r01 * s1;
+    z2 = r23 * co3;
+    if (!(fabsf(z0) > 5.0f || fabsf(z1) > 5.0f || fabsf(z2) > 5.0f) || a == 0xFFFFu) break;
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------- wave helpers
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// atomicAdd(base[key], 1) for every active lane, one atomic per distinct key
+// in the wavefront (ballot match loop); returns the lane's slot = old value +
+// rank among the lanes of the same key.
+__device__ __forceinline__ uint32_t agg_add(uint32_t *base, uint32_t key) {
+  const int lane = __lane_id();
+  unsigned long long todo = __ballot(1);
+  uint32_t res = 0;
+  while (todo) {
+    const int leader = __ffsll(todo) - 1;
+    const uint32_t lkey = __shfl(key, leader);
+    const unsigned long long same = __ballot(key == lkey) & todo;
+    uint32_t old = 0;
+    if (lane == leader) old = atomicAdd(base + lkey, (uint32_t)__popcll(same));
+    old = __shfl(old, leader);
+    if ((same >> lane) & 1ull) res = old + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return res;
+}
+
+__device__ __forceinline__ void agg_add_noret(uint32_t *base, uint32_t key) {
+  const int lane = __lane_id();
+  unsigned long long todo = __ballot(1);
+  while (todo) {
+    const int leader = __ffsll(todo) - 1;
+    const uint32_t lkey = __shfl(key, leader);
+    const unsigned long long same = __ballot(key == lkey) & todo;
+    if (lane == leader) atomicAdd(base + lkey, (uint32_t)__popcll(same));
+    todo &= ~same;
+  }
+}
+
+// ---------------------------------------------------------------- grid
+// Cell coordinate along one axis: floor(x / csl) + cps/2 as ccp's
+// calc_indexes (sugrid.cu:26-34), clamped into the grid (ccp has no clamp).
+__device__ __forceinline__ int cell_coord(float x, const KParams &p, bool &outside) {
+  int g = (int)floorf(x / p.csl) + p.half;
+  if (g < 0) { g = 0; outside = true; }
+  if (g >= p.cps) { g = p.cps - 1; outside = true; }
+  return g;
+}
+
+__device__ __forceinline__ void bin_position(const KParams &p, const KPtrs &q, uint32_t i, float x, float y, float z) {
+  bool outside = false;
+  const int gx = cell_coord(x, p, outside), gy = cell_coord(y, p, outside), gz = cell_coord(z, p, outside);
+  const uint32_t key = (uint32_t)((gz * p.cps + gy) * p.cps + gx);
+  const uint32_t slot = agg_add(q.count, key);
+  if (slot < (uint32_t)p.cap) q.bucket[(size_t)key * p.cap + slot] = i;
+  agg_add_noret(q.blocksum, key / SCAN_BLOCK);
+  if (outside) atomicAdd(q.flags + FLAG_OUTSIDE, 1u);
+  if (!(isfinite(x) && isfinite(y) && isfinite(z))) atomicAdd(q.flags + FLAG_NONFINITE, 1u);
+}
+
+// ---------------------------------------------------------------- forces
+// Non-bonded pair force on bead i from the sorted entry o, accumulated into f.
+// CCP: Wang-Frenkel e_p (18 d^4 - 96 d^2 + 96)/d^8 (calc_ppf<DST>,
+// chrsim.cu:173-185) cut at aco for het-het pairs and at rco otherwise.
+// ICG: soft repulsion 128 (3 rco - 3 d) (calc_ppf<ICG>, :188-198).
+// LJ: WCA for d < 2^(1/6) sigma, LJ tail eps_het between het beads to rc_lj.
+// Pairs with |i - j| <= 1 are skipped (calc_cell_srf, :250).
+template <int MODE>
+__device__ __forceinline__ void pair_force(const KParams &p, float rx, float ry, float rz, uint32_t wi, float4 o,
+                                           float &fx, float &fy, float &fz) {
+  const uint32_t wj = wbits(o);
+  const uint32_t i = wi & ID_MASK, j = wj & ID_MASK;
+  const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  const bool nb = (j > i + 1u) || (i > j + 1u);
+  const bool hhi = (wi & wj & W_HET) != 0u;
+  float fac;
+  if constexpr (MODE == MODE_CCP) {
+    const float u = fast_rcp(d2);
+    const float rc2 = hhi ? p.aco2 : p.rco2;
+    fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
+    fac = (nb && d2 <= rc2) ? fac : 0.0f;
+  } else if constexpr (MODE == MODE_ICG) {
+    const float d = sqrtf(d2);
+    fac = 128.0f * (3.0f * p.rco - 3.0f * d);
+    fac = (nb && d <= p.rco) ? fac : 0.0f;
+  } else {
+    const float u = fast_rcp(d2);
+    const float s2 = p.sig2 * u;
+    const float s6 = s2 * s2 * s2;
+    const float base = 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
+    const bool wca = nb && d2 < p.rm2;
+    const bool tail = nb && hhi && !wca && d2 < p.rc_lj2;
+    fac = wca ? p.eps_wca * base : (tail ? p.eps_het * base : 0.0f);
+  }
+  fx += fac * dx;
+  fy += fac * dy;
+  fz += fac * dz;
+}
+
+// LBS binding e_l 8/(3 lco^2) ((d/lco)^6 - 1) (calc_lpf<DST>, chrsim.cu:201-212).
+__device__ __forceinline__ void lbs_force(const KParams &p, float rx, float ry, float rz, float4 o, float &fx,
+                                          float &fy, float &fz) {
+  const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  const float n2 = d2 * p.inv_lco2;
+  float fac = p.lfac * (n2 * n2 * n2 - 1.0f);
+  fac = (d2 <= p.lco2) ? fac : 0.0f;
+  fx += fac * dx;
+  fy += fac * dy;
+  fz += fac * dz;
+}
+
+// Bonds and bending on bead i (calc_bf, chrsim.cu:13-76); bond existence
+// comes from the W_B* bits, positions from the bead-ordered array.
+__device__ __forceinline__ void bonded_force(const KParams &p, const float4 *__restrict__ pos, uint32_t i,
+                                             uint32_t w, float rx, float ry, float rz, float &fx, float &fy,
+                                             float &fz) {
+  float v0x = 0, v0y = 0, v0z = 0, v1x = 0, v1y = 0, v1z = 0, v2x = 0, v2y = 0, v2z = 0, v3x = 0, v3y = 0, v3z = 0;
+  float il0 = 0, il1 = 0, il2 = 0, il3 = 0;
+  if (w & W_B1) {
+    const float4 a = pos[i - 1];
+    v1x = rx - a.x; v1y = ry - a.y; v1z = rz - a.z;
+    il1 = rsqrtf(v1x * v1x + v1y * v1y + v1z * v1z);
+    if (w & W_B0) {
+      const float4 b = pos[i - 2];
+      v0x = a.x - b.x; v0y = a.y - b.y; v0z = a.z - b.z;
+      il0 = rsqrtf(v0x * v0x + v0y * v0y + v0z * v0z);
+    }
+  }
+  if (w & W_B2) {
+    const float4 a = pos[i + 1];
+    v2x = a.x - rx; v2y = a.y - ry; v2z = a.z - rz;
+    il2 = rsqrtf(v2x * v2x + v2y * v2y + v2z * v2z);
+    if (w & W_B3) {
+      const float4 b = pos[i + 2];
+      v3x = b.x - a.x; v3y = b.y - a.y; v3z = b.z - a.z;
+      il3 = rsqrtf(v3x * v3x + v3y * v3y + v3z * v3z);
+    }
+  }
+  if (p.bond_type == 0) {
+    const float c2 = p.k_e * (1.0f - p.l_0 * il2), c1 = p.k_e * (1.0f - p.l_0 * il1);
+    fx += c2 * v2x - c1 * v1x;
+    fy += c2 * v2y - c1 * v1y;
+    fz += c2 * v2z - c1 * v1z;
+  } else {
+    // FENE K b / (1 - b^2/R0^2) plus WCA between bonded beads
+    if (w & W_B2) {
+      const float d2 = v2x * v2x + v2y * v2y + v2z * v2z;
+      float c = p.fene_k / fmaxf(1.0f - d2 / p.fene_r02, 0.01f);
+      if (d2 < p.rm2) {
+        const float u = fast_rcp(d2), s2 = p.sig2 * u, s6 = s2 * s2 * s2;
+        c -= p.eps_wca * 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
+      }
+      fx += c * v2x; fy += c * v2y; fz += c * v2z;
+    }
+    if (w & W_B1) {
+      const float d2 = v1x * v1x + v1y * v1y + v1z * v1z;
+      float c = -p.fene_k / fmaxf(1.0f - d2 / p.fene_r02, 0.01f);
+      if (d2 < p.rm2) {
+        const float u = fast_rcp(d2), s2 = p.sig2 * u, s6 = s2 * s2 * s2;
+        c += p.eps_wca * 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
+      }
+      fx += c * v1x; fy += c * v1y; fz += c * v1z;
+    }
+  }
+  if (p.bend_all || (w & W_HET)) {
+    const float cs0 = (v1x * v0x + v1y * v0y + v1z * v0z) * il1 * il0;
+    const float cs1 = (v2x * v1x + v2y * v1y + v2z * v1z) * il2 * il1;
+    const float cs2 = (v3x * v2x + v3y * v2y + v3z * v2z) * il3 * il2;
+    const float a0 = il1 * il0, a1 = il1 * il2, a3 = il2 * il3;
+    const float q1 = il1 * il1, q2 = il2 * il2;
+    // k_b [ a0 v0 - cs0 q1 v1 + a1 v2 - cs1 q1 v1 - a1 v1 + cs1 q2 v2 - a3 v3 + cs2 q2 v2 ]
+    const float c0 = p.k_b * a0;
+    const float c1 = -p.k_b * (cs0 * q1 + cs1 * q1 + a1);
+    const float c2 = p.k_b * (a1 + cs1 * q2 + cs2 * q2);
+    const float c3 = -p.k_b * a3;
+    fx += c0 * v0x + c1 * v1x + c2 * v2x + c3 * v3x;
+    fy += c0 * v0y + c1 * v1y + c2 * v2y + c3 * v3y;
+    fz += c0 * v0z + c1 * v1z + c2 * v2z + c3 * v3z;
+  }
+}
+
+// Confinement (calc_cf<DST>, chrsim.cu:91-144, sphere + opening + bleb;
+// calc_cf<ICG>, :147-170, sphere only) with the Wang-Frenkel wall
+// (calc_wpf, :79-88) or, for LJ, WCA plus a lamina tail on LAD beads.
+template <int MODE>
+__device__ __forceinline__ void wall_force(const KParams &p, float rx, float ry, float rz, uint32_t w, float &fx,
+                                           float &fy, float &fz) {
+  float vx, vy, vz;
+  if constexpr (MODE == MODE_ICG) {
+    const float d_r = sqrtf(rx * rx + ry * ry + rz * rz);
+    const float s = -(p.R_n / d_r - 1.0f);
+    vx = rx * s; vy = ry * s; vz = rz * s;
+  } else {
+    if (rz < p.nod) {
+      float d_r = sqrtf(rx * rx + ry * ry + rz * rz);
+      if (rz / d_r < p.noc) {
+        const float s = -(p.R_n / d_r - 1.0f);
+        vx = rx * s; vy = ry * s; vz = rz * s;
+      } else {
+        d_r = sqrtf(rx * rx + ry * ry);
+        const float s = -(p.R_o / d_r - 1.0f);
+        vx = rx * s; vy = ry * s; vz = rz - p.nod;
+      }
+    } else {
+      const float qz = rz - (p.nod + p.bod);
+      float d_r = sqrtf(rx * rx + ry * ry + qz * qz);
+      if (-qz / d_r < p.boc) {
+        const float s = -(p.R_b / d_r - 1.0f);
+        vx = rx * s; vy = ry * s; vz = qz * s;
+      } else {
+        d_r = sqrtf(rx * rx + ry * ry);
+        const float s = -(p.R_o / d_r - 1.0f);
+        vx = rx * s; vy = ry * s; vz = qz + p.bod;
+      }
+    }
+  }
+  const float dw2 = vx * vx + vy * vy + vz * vz;
+  if (!isfinite(dw2)) return;
+  float fac;
+  if constexpr (MODE != MODE_LJ) {
+    if (dw2 > p.rco2) return;
+    const float u = fast_rcp(dw2);
+    fac = u * u * (18.0f + u * (96.0f * u - 96.0f));
+  } else {
+    const float u = fast_rcp(dw2);
+    const float s2 = p.sig2 * u, s6 = s2 * s2 * s2;
+    const float base = 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
+    if (dw2 < p.rm2) fac = p.eps_wall * base;
+    else if ((w & W_LAD) && dw2 < p.rc_lam2) fac = p.eps_lam * base;
+    else return;
+  }
+  fx += fac * vx;
+  fy += fac * vy;
+  fz += fac * vz;
+}
+
+// ---------------------------------------------------------------- kernels
+// STAGE 0: write F(pos) by bead index. STAGE 1: Heun predictor
+// (exec_RK_1, chrsim.cu:364-388) er = r + f dt + rn, keeps
+// r + f dt/2 + rn. STAGE 2: corrector (exec_RK_2, :391-416)
+// r = (r + f dt/2 + rn) + ef dt/2. Stages 1 and 2 bin their output.
+template <int MODE, int STAGE>
+__global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
+  const int t = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (t >= p.N) return;
+  const float4 me = q.spos[t];
+  const uint32_t w = wbits(me);
+  const uint32_t i = w & ID_MASK;
+  const float rx = me.x, ry = me.y, rz = me.z;
+  bool outside = false;
+  const int cx = cell_coord(rx, p, outside), cy = cell_coord(ry, p, outside), cz = cell_coord(rz, p, outside);
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  const bool lbs = (MODE != MODE_ICG) && (p.n_l > 0) && (w & W_LAD);
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= p.cps) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= p.cps) continue;
+      const int row = (z * p.cps + y) * p.cps;
+      const int c0 = row + max(cx - 1, 0), c1 = row + min(cx + 1, p.cps - 1) + 1;
+      const uint32_t lo = q.cstart[c0], hi = q.cstart[c1];
+      for (uint32_t k = lo; k < hi; ++k) pair_force<MODE>(p, rx, ry, rz, w, q.spos[k], fx, fy, fz);
+      if (lbs) {
+        const uint32_t llo = q.lstart[c0], lhi = q.lstart[c1];
+        for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
+      }
+    }
+  }
+  bonded_force(p, q.pos, i, w, rx, ry, rz, fx, fy, fz);
+  wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
+  if (STAGE != 0 && p.fcap2 > 0.0f) {
+    // optional safeguard of the WCA/LJ set: |F| <= fcap
+    const float m2 = fx * fx + fy * fy + fz * fz;
+    if (m2 > p.fcap2) {
+      const float s = sqrtf(p.fcap2 / m2);
+      fx *= s; fy *= s; fz *= s;
+    }
+  }
+
+  if constexpr (STAGE == 0) {
+    q.out0[i] = make_float4(fx, fy, fz, 0.0f);
+  } else if constexpr (STAGE == 1) {
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (p.sd > 0.0f) {
+      float z0, z1, z2;
+      noise3(p.seed_lo, p.seed_hi, i, *q.step, z0, z1, z2);
+      nx = p.sd * z0; ny = p.sd * z1; nz = p.sd * z2;
+    }
+    const float ex = rx + fx * p.dt + nx, ey = ry + fy * p.dt + ny, ez = rz + fz * p.dt + nz;
+    q.out0[i] = make_float4(ex, ey, ez, me.w);
+    q.pstar[i] = make_float4(rx + fx * p.half_dt + nx, ry + fy * p.half_dt + ny, rz + fz * p.half_dt + nz, 0.0f);
+    bin_position(p, q, i, ex, ey, ez);
+  } else {
+    const float4 ps = q.pstar[i];
+    const float nx = ps.x + fx * p.half_dt, ny = ps.y + fy * p.half_dt, nz = ps.z + fz * p.half_dt;
+    q.out0[i] = make_float4(nx, ny, nz, me.w);
+    bin_position(p, q, i, nx, ny, nz);
+  }
+}
+
+// Binning of an index-ordered array (initial build and LBS grid).
+__global__ __launch_bounds__(BEAD_BLOCK) void k_bin(KParams p, int n_obj, const float4 *pos, KPtrs q) {
+  const int i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (i >= n_obj) return;
+  const float4 v = pos[i];
+  bin_position(p, q, (uint32_t)i, v.x, v.y, v.z);
+}
+
+// Per-cell scan + deterministic order + gather (replaces ccp's
+// calc_indexes/SortPairs/set_cells_empty/find_cells_limits sequence).
+__global__ __launch_bounds__(SCAN_BLOCK) void k_order(KParams p, uint32_t *count, const uint32_t *bs_read,
+                                                      uint32_t *bs_clear, const uint32_t *bucket, const float4 *pos,
+                                                      float4 *spos, uint32_t *soi, uint32_t *cstart,
+                                                      uint32_t *flags, uint64_t *step_inc) {
+  __shared__ uint32_t red[SCAN_BLOCK / 64];
+  __shared__ uint32_t wsum[SCAN_BLOCK / 64];
+  const int b = blockIdx.x, tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
+  const int c = b * SCAN_BLOCK + tid;
+  uint32_t part = 0;
+  for (int k = tid; k < b; k += SCAN_BLOCK) part += bs_read[k];
+  part = wave_sum(part);
+  if (lane == 0) red[wid] = part;
+  const uint32_t cnt = (c < p.n_c) ? count[c] : 0u;
+  const uint32_t incl = wave_incl_scan(cnt);
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  uint32_t prefix = 0, woff = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_BLOCK / 64; ++k) {
+    prefix += red[k];
+    if (k < wid) woff += wsum[k];
+  }
+  const uint32_t start = prefix + woff + incl - cnt;
+  if (c < p.n_c) {
+    cstart[c] = start;
+    count[c] = 0u;
+  }
+  if (c == p.n_c - 1) cstart[p.n_c] = start + cnt;
+  if (tid == 0) {
+    bs_clear[b] = 0u;
+    if (b == 0 && step_inc) *step_inc += 1ull;
+  }
+  if (c >= p.n_c || cnt == 0u) return;
+  uint32_t m = cnt;
+  if (m > (uint32_t)p.cap) {
+    atomicAdd(flags + FLAG_OVERFLOW, 1u);
+    m = (uint32_t)p.cap;
+  }
+  const uint32_t *bk = bucket + (size_t)c * p.cap;
+  for (uint32_t k = 0; k < m; ++k) {
+    const uint32_t e = bk[k];
+    uint32_t rank = 0;
+    for (uint32_t s = 0; s < m; ++s) rank += (bk[s] < e) ? 1u : 0u;
+    spos[start + rank] = pos[e];
+    if (soi) soi[start + rank] = e;
+  }
+}
+
+// Contact counts: pairs i < j with d < rc, FP32 without fused operations so
+// the decision matches the oracle bit for bit.
+__global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4 *spos, const uint32_t *cstart,
+                                                         uint32_t *map) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (t >= p.N) return;
+  const float4 me = spos[t];
+  const uint32_t i = wbits(me) & ID_MASK;
+  bool outside = false;
+  const int cx = cell_coord(me.x, p, outside), cy = cell_coord(me.y, p, outside), cz = cell_coord(me.z, p, outside);
+  const uint32_t nb = (uint32_t)p.contact_bins, bi = i / (uint32_t)p.contact_bin_size;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= p.cps) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= p.cps) continue;
+      const int row = (z * p.cps + y) * p.cps;
+      const uint32_t lo = cstart[row + max(cx - 1, 0)], hi = cstart[row + min(cx + 1, p.cps - 1) + 1];
+      for (uint32_t k = lo; k < hi; ++k) {
+        const float4 o = spos[k];
+        const uint32_t j = wbits(o) & ID_MASK;
+        if (j <= i) continue;
+        const float dx = me.x - o.x, dy2 = me.y - o.y, dz2 = me.z - o.z;
+        const float xx = dx * dx, yy = dy2 * dy2, zz = dz2 * dz2;
+        const float d2 = (xx + yy) + zz;
+        if (d2 < p.contact_rc2) {
+          const uint32_t bj = j / (uint32_t)p.contact_bin_size;
+          if (bi < nb && bj < nb) {
+            atomicAdd(map + (size_t)bi * nb + bj, 1u);
+            atomicAdd(map + (size_t)bj * nb + bi, 1u);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Non-bonded pairs (|i - j| > 1) closer than dmin (particle_overlaps,
+// chrsim.cu:805-820), each pair counted once.
+__global__ __launch_bounds__(BEAD_BLOCK) void k_overlaps(KParams p, const float4 *spos, const uint32_t *cstart,
+                                                         float dmin2, unsigned long long *count) {
+  const int t = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  uint32_t n = 0;
+  if (t < p.N) {
+    const float4 me = spos[t];
+    const uint32_t i = wbits(me) & ID_MASK;
+    bool outside = false;
+    const int cx = cell_coord(me.x, p, outside), cy = cell_coord(me.y, p, outside), cz = cell_coord(me.z, p, outside);
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int z = cz + dz;
+      if (z < 0 || z >= p.cps) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int y = cy + dy;
+        if (y < 0 || y >= p.cps) continue;
+        const int row = (z * p.cps + y) * p.cps;
+        const uint32_t lo = cstart[row + max(cx - 1, 0)], hi = cstart[row + min(cx + 1, p.cps - 1) + 1];
+        for (uint32_t k = lo; k < hi; ++k) {
+          const float4 o = spos[k];
+          const uint32_t j = wbits(o) & ID_MASK;
+          const float dx = me.x - o.x, dy2 = me.y - o.y, dz2 = me.z - o.z;
+          if (j > i + 1u && dx * dx + dy2 * dy2 + dz2 * dz2 < dmin2) ++n;
+        }
+      }
+    }
+  }
+  const uint32_t tot = wave_sum(n);
+  if (__lane_id() == 0 && tot) atomicAdd(count, (unsigned long long)tot);
+}
+
+__global__ __launch_bounds__(BEAD_BLOCK) void k_noise(KParams p, uint64_t step, float4 *out) {
+  const int i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (i >= p.N) return;
+  float z0, z1, z2;
+  const uint32_t a = noise3(p.seed_lo, p.seed_hi, (uint32_t)i, step, z0, z1, z2);
+  out[i] = make_float4(z0, z1, z2, __uint_as_float(a));
+}
+
+inline unsigned blocks_for(int n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+} // namespace
+
+void launch_bin(hipStream_t s, const KParams &p, int n_obj, const float4 *pos, uint32_t *count, uint32_t *blocksum,
+                uint32_t *bucket, uint32_t *flags) {
+  if (n_obj <= 0) return;
+  KPtrs q{};
+  q.count = count;
+  q.blocksum = blocksum;
+  q.bucket = bucket;
+  q.flags = flags;
+  k_bin<<<blocks_for(n_obj, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, n_obj, pos, q);
+}
+
+void launch_order(hipStream_t s, const KParams &p, int n_obj, uint32_t *count, const uint32_t *blocksum_read,
+                  uint32_t *blocksum_clear, const uint32_t *bucket, const float4 *pos, float4 *spos, uint32_t *soi,
+                  uint32_t *cstart, uint32_t *flags, uint64_t *step_inc) {
+  (void)n_obj;
+  k_order<<<(unsigned)p.nblk, SCAN_BLOCK, 0, s>>>(p, count, blocksum_read, blocksum_clear, bucket, pos, spos, soi,
+                                                  cstart, flags, step_inc);
+}
+
+template <int MODE> static void force_mode(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
+  const unsigned g = blocks_for(p.N, BEAD_BLOCK);
+  if (stage == 0) k_force<MODE, 0><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else if (stage == 1) k_force<MODE, 1><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else k_force<MODE, 2><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+}
+
+void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q) {
+  if (p.N <= 0) return;
+  if (mode == MODE_CCP) force_mode<MODE_CCP>(s, stage, p, q);
+  else if (mode == MODE_ICG) force_mode<MODE_ICG>(s, stage, p, q);
+  else force_mode<MODE_LJ>(s, stage, p, q);
+}
+
+void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map) {
+  if (p.N <= 0) return;
+  k_contacts<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, map);
+}
+
+void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, float dmin2,
+                     unsigned long long *count) {
+  if (p.N <= 0) return;
+  k_overlaps<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, dmin2, count);
+}
+
+void launch_noise(hipStream_t s, const KParams &p, uint64_t step, float4 *out) {
+  if (p.N <= 0) return;
+  k_noise<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, step, out);
+}
+
+} // namespace ccpmi

@@ -1,0 +1,83 @@
+// Internal interface between the host side of libccpmi and its gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ccpmi {
+
+// Bits of the 32-bit word stored in .w of every bead float4:
+// bead index, type flags, and which of ccp's four bond vectors around the
+// bead exist (calc_bf's lim_l/lim_u test, chrsim.cu:43-56).
+constexpr uint32_t ID_BITS = 26;
+constexpr uint32_t ID_MASK = (1u << ID_BITS) - 1u;
+constexpr uint32_t W_HET = 1u << 26;
+constexpr uint32_t W_LAD = 1u << 27;
+constexpr uint32_t W_B0 = 1u << 28; // r[i-1] - r[i-2]
+constexpr uint32_t W_B1 = 1u << 29; // r[i]   - r[i-1]
+constexpr uint32_t W_B2 = 1u << 30; // r[i+1] - r[i]
+constexpr uint32_t W_B3 = 1u << 31; // r[i+2] - r[i+1]
+
+// cells per scan block of the order kernel
+constexpr int SCAN_BLOCK = 256;
+// threads per block of the bead kernels
+constexpr int BEAD_BLOCK = 256;
+
+// device error flags
+enum : int { FLAG_OVERFLOW = 0, FLAG_OUTSIDE = 1, FLAG_NONFINITE = 2, FLAG_COUNT = 4 };
+
+enum Mode : int { MODE_CCP = 0, MODE_ICG = 1, MODE_LJ = 2 };
+
+struct KParams {
+  int N, n_l;
+  int cps, half, n_c, cap, nblk;
+  float csl;
+  // nucleus geometry (cngeom, chrdat.cuh:48-69)
+  float R_n, R_o, R_b, noc, boc, nod, bod;
+  // bonds and bending
+  float k_e, l_0, k_b;
+  int bend_all, bond_type;
+  float fene_k, fene_r02;
+  // Wang-Frenkel pair set (ccp)
+  float e_p, rco, rco2, aco2;
+  // LBS binding
+  float lfac, lco2, inv_lco2;
+  // WCA/LJ set
+  float eps_wca, sig2, rm2, eps_het, rc_lj2, eps_wall, eps_lam, rc_lam2;
+  // integrator
+  float dt, half_dt, sd;
+  float fcap2; // squared force cap of the integrator (0 = off)
+  uint32_t seed_lo, seed_hi;
+  // contacts
+  float contact_rc2;
+  int contact_bin_size, contact_bins;
+};
+
+struct KPtrs {
+  const float4 *spos;     // bead positions sorted by cell
+  const uint32_t *cstart; // n_c + 1 cell starts into spos
+  const float4 *lspos;    // LBS positions sorted by cell
+  const uint32_t *lstart; // n_c + 1
+  const float4 *pos;      // bead positions of the evaluated stage, by bead index
+  float4 *out0;           // stage 1: er, stage 2: new r, stage 0: force
+  float4 *pstar;          // r + f dt/2 + noise (written by stage 1, read by stage 2)
+  uint32_t *count;        // n_c per-cell counters (zero between builds)
+  uint32_t *blocksum;     // nblk per-scan-block counters being accumulated
+  uint32_t *bucket;       // n_c * cap unsorted cell members
+  uint32_t *flags;        // FLAG_COUNT error counters
+  const uint64_t *step;   // global step counter (noise counter)
+};
+
+// Launchers (kernels.hip). All are asynchronous on `s`.
+void launch_bin(hipStream_t s, const KParams &p, int n_obj, const float4 *pos, uint32_t *count, uint32_t *blocksum,
+                uint32_t *bucket, uint32_t *flags);
+void launch_order(hipStream_t s, const KParams &p, int n_obj, uint32_t *count, const uint32_t *blocksum_read,
+                  uint32_t *blocksum_clear, const uint32_t *bucket, const float4 *pos, float4 *spos, uint32_t *soi,
+                  uint32_t *cstart, uint32_t *flags, uint64_t *step_inc);
+void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q);
+void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map);
+void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, float dmin2,
+                     unsigned long long *count);
+void launch_noise(hipStream_t s, const KParams &p, uint64_t step, float4 *out);
+
+} // namespace ccpmi
