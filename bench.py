@@ -181,7 +181,7 @@ def main():
         },
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=float), flush=True)
     sim.close()
 
 

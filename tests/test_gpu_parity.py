@@ -74,7 +74,8 @@ def test_forces_match_oracle(case):
     err = np.abs(got - want).max()
     err32 = np.abs(w32 - want).max()
     step(f"  max|f|={scale:.3g} gpu err={err:.3g} oracle fp32 err={err32:.3g}")
-    assert err <= 1e-5 * scale + 1e-4
+    # FP32 bar: no worse than 3x an FP32 evaluation of the same formulas
+    assert err <= 3 * err32 + 1e-6 * scale
     sim.close()
 
 

@@ -250,6 +250,8 @@ struct ccp_sim {
     return k;
   }
   void build_grid(const float4 *pos, int n_obj, float4 *out_spos, uint32_t *out_cstart, uint32_t *out_soi);
+  bool take_overflow();
+  void grow_capacity();
   void ensure_grids();
   void sync_step();
   void launch_stage(int mode, int stage);
@@ -538,15 +540,44 @@ void ccp_sim::build_grid(const float4 *pos, int n_obj, float4 *out_spos, uint32_
   clean_bs ^= 1;
 }
 
+// Overflow flag of the last builds (synchronizes the stream); clears it.
+bool ccp_sim::take_overflow() {
+  uint32_t f = 0;
+  hip_check(hipMemcpyAsync(&f, flags.p + FLAG_OVERFLOW, 4, hipMemcpyDeviceToHost, stream));
+  hip_check(hipStreamSynchronize(stream));
+  if (f) hip_check(hipMemset(flags.p + FLAG_OVERFLOW, 0, 4));
+  return f != 0;
+}
+
+// Double the cell capacity (bucket storage); captured graphs refer to the
+// old buffer and are dropped.
+void ccp_sim::grow_capacity() {
+  if (cap >= 4096) throw Error(CCP_ERR_STATE, "cell_capacity exceeded; raise cell_capacity");
+  hip_check(hipStreamSynchronize(stream));
+  for (auto &gm : graphs)
+    for (auto &g : gm)
+      if (g) {
+        hip_check(hipGraphExecDestroy(g));
+        g = nullptr;
+      }
+  cap *= 2;
+  kp.cap = (int)cap;
+  bucket.free_();
+  bucket.alloc((size_t)n_c * cap);
+  grid_valid = lgrid_valid = false;
+}
+
+// Cell lists of the LBS sites and of the current positions; grows the
+// capacity until every cell fits (not used inside the step loop).
 void ccp_sim::ensure_grids() {
-  if (!lgrid_valid) {
-    build_grid(lr.p, (int)n_l, lspos.p, lstart.p, nullptr);
-    lgrid_valid = true;
+  if (grid_valid && lgrid_valid) return;
+  for (;;) {
+    if (!lgrid_valid) build_grid(lr.p, (int)n_l, lspos.p, lstart.p, nullptr);
+    if (!grid_valid) build_grid(r.p, (int)N, spos.p, cstart.p, nullptr);
+    if (!take_overflow()) break;
+    grow_capacity();
   }
-  if (!grid_valid) {
-    build_grid(r.p, (int)N, spos.p, cstart.p, nullptr);
-    grid_valid = true;
-  }
+  grid_valid = lgrid_valid = true;
 }
 
 void ccp_sim::sync_step() {
@@ -842,6 +873,9 @@ void ccp_sim::generate_initial_condition() {
     progress((float)(100.0 * (ipo - po) / ipo));
     if (++rounds > 10000) throw Error(CCP_ERR_STATE, "unable to remove bead overlaps");
     run_steps(spf, MODE_ICG);
+    if (take_overflow()) { // clumps of the raw walk: widen the cells' storage and go on
+      grow_capacity();
+    }
     check_flags();
     po = count_overlaps(MIS);
   }
@@ -1274,7 +1308,11 @@ int ccp_compute_forces(ccp_sim *h, int ff, float *f_xyz, size_t n) {
 int ccp_cell_list(ccp_sim *h, uint32_t *sci, uint32_t *soi, uint32_t *beg, uint32_t *end, size_t n, size_t n_c) {
   return guarded(h, [&] {
     need(n == h->N && n_c == h->n_c && sci && soi && beg && end, "cell list size mismatch");
-    h->build_grid(h->r.p, (int)h->N, h->spos.p, h->cstart.p, h->soi.p);
+    for (;;) {
+      h->build_grid(h->r.p, (int)h->N, h->spos.p, h->cstart.p, h->soi.p);
+      if (!h->take_overflow()) break;
+      h->grow_capacity();
+    }
     h->grid_valid = true;
     std::vector<uint32_t> cs(h->n_c + 1);
     hip_check(hipMemcpyAsync(soi, h->soi.p, 4 * n, hipMemcpyDeviceToHost, h->stream));
