@@ -332,26 +332,46 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
   const float rx = me.x, ry = me.y, rz = me.z;
   bool outside = false;
   const int cx = cell_coord(rx, p, outside), cy = cell_coord(ry, p, outside), cz = cell_coord(rz, p, outside);
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  const bool lbs = (MODE != MODE_ICG) && (p.n_l > 0) && (w & W_LAD);
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = cz + dz;
-    if (z < 0 || z >= p.cps) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = cy + dy;
-      if (y < 0 || y >= p.cps) continue;
-      const int row = (z * p.cps + y) * p.cps;
-      const int c0 = row + max(cx - 1, 0), c1 = row + min(cx + 1, p.cps - 1) + 1;
-      const uint32_t lo = q.cstart[c0], hi = q.cstart[c1];
-      for (uint32_t k = lo; k < hi; ++k) pair_force<MODE>(p, rx, ry, rz, w, q.spos[k], fx, fy, fz);
-      if (lbs) {
-        const uint32_t llo = q.lstart[c0], lhi = q.lstart[c1];
-        for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
-      }
-    }
+
+  // slot ranges of the 9 neighbour rows (x-1..x+1 of rows y+dy, z+dz), all
+  // loaded up front so the 18 start lookups are in flight together
+  uint32_t lo[9], hi[9];
+  int rc0[9], rc1[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
+    const bool ok = z >= 0 && z < p.cps && y >= 0 && y < p.cps;
+    const int row = (z * p.cps + y) * p.cps;
+    rc0[r] = ok ? row + max(cx - 1, 0) : 0;
+    rc1[r] = ok ? row + min(cx + 1, p.cps - 1) + 1 : 0;
+    lo[r] = q.cstart[rc0[r]];
+    hi[r] = q.cstart[rc1[r]];
   }
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   bonded_force(p, q.pos, i, w, rx, ry, rz, fx, fy, fz);
   wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
+
+  // candidates four at a time: independent loads per iteration
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    uint32_t k = lo[r];
+    const uint32_t h = hi[r];
+    for (; k + 4u <= h; k += 4u) {
+      const float4 a = q.spos[k], b = q.spos[k + 1], c = q.spos[k + 2], d = q.spos[k + 3];
+      pair_force<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
+      pair_force<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
+      pair_force<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
+      pair_force<MODE>(p, rx, ry, rz, w, d, fx, fy, fz);
+    }
+    for (; k < h; ++k) pair_force<MODE>(p, rx, ry, rz, w, q.spos[k], fx, fy, fz);
+  }
+  if (MODE != MODE_ICG && p.n_l > 0 && (w & W_LAD)) {
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const uint32_t llo = q.lstart[rc0[r]], lhi = q.lstart[rc1[r]];
+      for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
+    }
+  }
   if (STAGE != 0 && p.fcap2 > 0.0f) {
     // optional safeguard of the WCA/LJ set: |F| <= fcap
     const float m2 = fx * fx + fy * fy + fz * fz;
@@ -392,12 +412,19 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_bin(KParams p, int n_obj, const 
 
 // Per-cell scan + deterministic order + gather (replaces ccp's
 // calc_indexes/SortPairs/set_cells_empty/find_cells_limits sequence).
+// The block's bucket entries are staged in LDS; each entry's rank within
+// its cell (number of smaller bead indices) is then found in parallel.
+constexpr int ORDER_LDS = 4096;
+
 __global__ __launch_bounds__(SCAN_BLOCK) void k_order(KParams p, uint32_t *count, const uint32_t *bs_read,
                                                       uint32_t *bs_clear, const uint32_t *bucket, const float4 *pos,
                                                       float4 *spos, uint32_t *soi, uint32_t *cstart,
                                                       uint32_t *flags, uint64_t *step_inc) {
-  __shared__ uint32_t red[SCAN_BLOCK / 64];
-  __shared__ uint32_t wsum[SCAN_BLOCK / 64];
+  __shared__ uint32_t red[SCAN_BLOCK / 64], wsum[SCAN_BLOCK / 64], msum[SCAN_BLOCK / 64];
+  __shared__ uint32_t s_id[ORDER_LDS];
+  __shared__ uint8_t s_cell[ORDER_LDS];
+  __shared__ uint32_t s_loff[SCAN_BLOCK + 1];
+  __shared__ uint32_t s_start[SCAN_BLOCK];
   const int b = blockIdx.x, tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
   const int c = b * SCAN_BLOCK + tid;
   uint32_t part = 0;
@@ -405,16 +432,30 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(KParams p, uint32_t *count
   part = wave_sum(part);
   if (lane == 0) red[wid] = part;
   const uint32_t cnt = (c < p.n_c) ? count[c] : 0u;
+  uint32_t m = cnt;
+  if (m > (uint32_t)p.cap) {
+    atomicAdd(flags + FLAG_OVERFLOW, 1u);
+    m = (uint32_t)p.cap;
+  }
   const uint32_t incl = wave_incl_scan(cnt);
-  if (lane == 63) wsum[wid] = incl;
+  const uint32_t mincl = wave_incl_scan(m);
+  if (lane == 63) {
+    wsum[wid] = incl;
+    msum[wid] = mincl;
+  }
   __syncthreads();
-  uint32_t prefix = 0, woff = 0;
+  uint32_t prefix = 0, woff = 0, moff = 0, mtotal = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_BLOCK / 64; ++k) {
     prefix += red[k];
-    if (k < wid) woff += wsum[k];
+    mtotal += msum[k];
+    if (k < wid) {
+      woff += wsum[k];
+      moff += msum[k];
+    }
   }
   const uint32_t start = prefix + woff + incl - cnt;
+  moff += mincl - m;
   if (c < p.n_c) {
     cstart[c] = start;
     count[c] = 0u;
@@ -424,19 +465,34 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(KParams p, uint32_t *count
     bs_clear[b] = 0u;
     if (b == 0 && step_inc) *step_inc += 1ull;
   }
-  if (c >= p.n_c || cnt == 0u) return;
-  uint32_t m = cnt;
-  if (m > (uint32_t)p.cap) {
-    atomicAdd(flags + FLAG_OVERFLOW, 1u);
-    m = (uint32_t)p.cap;
-  }
   const uint32_t *bk = bucket + (size_t)c * p.cap;
-  for (uint32_t k = 0; k < m; ++k) {
-    const uint32_t e = bk[k];
-    uint32_t rank = 0;
-    for (uint32_t s = 0; s < m; ++s) rank += (bk[s] < e) ? 1u : 0u;
-    spos[start + rank] = pos[e];
-    if (soi) soi[start + rank] = e;
+  if (mtotal <= (uint32_t)ORDER_LDS) {
+    s_loff[tid] = moff;
+    s_start[tid] = start;
+    if (tid == SCAN_BLOCK - 1) s_loff[SCAN_BLOCK] = moff + m;
+    for (uint32_t k = 0; k < m; ++k) {
+      s_id[moff + k] = bk[k];
+      s_cell[moff + k] = (uint8_t)tid;
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < mtotal; e += SCAN_BLOCK) {
+      const uint32_t l = s_cell[e], id = s_id[e];
+      const uint32_t b0 = s_loff[l], b1 = s_loff[l + 1];
+      uint32_t rank = 0;
+      for (uint32_t s = b0; s < b1; ++s) rank += (s_id[s] < id) ? 1u : 0u;
+      const uint32_t dst = s_start[l] + rank;
+      spos[dst] = pos[id];
+      if (soi) soi[dst] = id;
+    }
+  } else {
+    // dense block (raw initial walks): rank each cell's entries in place
+    for (uint32_t k = 0; k < m; ++k) {
+      const uint32_t e = bk[k];
+      uint32_t rank = 0;
+      for (uint32_t s = 0; s < m; ++s) rank += (bk[s] < e) ? 1u : 0u;
+      spos[start + rank] = pos[e];
+      if (soi) soi[start + rank] = e;
+    }
   }
 }
 
