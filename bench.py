@@ -75,6 +75,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_force_lj_n131072_traffic.txt"))
+    ap.add_argument("--param", action="append", default=[], metavar="KEY=VALUE",
+                    help="override a simulation parameter (e.g. fine_cell_size=1.25)")
     return ap.parse_args()
 
 
@@ -124,7 +126,11 @@ def main():
     import tfm_amd
 
     print(f"[bench] N={N} R_n={R_N:.3f}: initial condition", file=sys.stderr, flush=True)
-    sim = tfm_amd.Simulation(PARAMS)
+    params = dict(PARAMS)
+    for kv in args.param:
+        k, v = kv.split("=", 1)
+        params[k] = v
+    sim = tfm_amd.Simulation(params)
     sim.generate_initial_condition()
     print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
     sim.run_steps(args.warmup)
@@ -171,6 +177,7 @@ def main():
                         "every 1000 steps into 1024x1024 uint32",
             "N": N, "chains": N_CHAINS, "R_n": round(R_N, 4), "cells_per_side": sim.info.cps,
             "contact_samples": n_samples, "contact_total": int(cmap.sum(dtype=np.uint64)),
+            "overrides": args.param,
         },
         "roofline": {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

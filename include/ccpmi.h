@@ -50,15 +50,17 @@ typedef struct ccp_info {
   uint32_t n_l;         /* lamina binding sites */
   uint32_t n_chains;
   uint32_t force_field; /* CCP_FF_CCP or CCP_FF_LJ */
-  uint32_t cps;         /* cells per side */
-  uint32_t n_c;         /* cells = cps^3 */
-  uint32_t cell_capacity;
+  uint32_t cps;         /* cells per side of the coarse grid (ccp's sugrid geometry) */
+  uint32_t n_c;         /* its cells = cps^3 (size of ccp_cell_list's beg/end) */
+  uint32_t cell_capacity; /* coarse grid bucket capacity */
+  uint32_t fine_cps;    /* cells per side of the fine grid of the step */
   uint32_t spf;         /* steps per frame */
   uint32_t fpf;         /* frames per file */
   uint32_t contact_bins;     /* contact map side */
   uint32_t contact_bin_size; /* beads per contact bin */
   uint32_t contact_interval; /* steps between contact samples (0 = off) */
-  float csl;            /* cell side length */
+  float csl;            /* coarse cell side (cell_size) */
+  float fine_csl;       /* fine cell side (fine_cell_size) */
   float R_n, R_o, R_b;
   float T, dt, sd;      /* temperature, timestep, noise standard deviation */
   float contact_cutoff;

@@ -38,10 +38,10 @@ class CcpError(RuntimeError):
 
 class Info(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
-        "N", "n_l", "n_chains", "force_field", "cps", "n_c", "cell_capacity", "spf", "fpf",
+        "N", "n_l", "n_chains", "force_field", "cps", "n_c", "cell_capacity", "fine_cps", "spf", "fpf",
         "contact_bins", "contact_bin_size", "contact_interval")] + \
-        [(n, ctypes.c_float) for n in ("csl", "R_n", "R_o", "R_b", "T", "dt", "sd", "contact_cutoff",
-                                             "force_cap")] + \
+        [(n, ctypes.c_float) for n in ("csl", "fine_csl", "R_n", "R_o", "R_b", "T", "dt", "sd",
+                                       "contact_cutoff", "force_cap")] + \
         [("seed", ctypes.c_uint64)]
 
 

@@ -158,8 +158,9 @@ struct ccp_sim {
   float dt = 0, sd = 0;
   uint64_t seed = 0;
   int ff = CCP_FF_CCP;
-  float csl = 0;
-  uint32_t cps = 0, n_c = 0, cap = 0, nblk = 0;
+  float csl = 0;   // coarse cell side: het grid and ccp-layout export grid
+  float csl_f = 0; // cell side of the all-bead grid of the step (fine when split)
+  bool split = false; // short-range pairs on the fine grid + het-het tail on the het grid
   std::vector<uint32_t> chain_off;
   std::string seq_file;
   int seq_mode = 0; // 0 ccp, 1 het/LAD generator
@@ -180,34 +181,73 @@ struct ccp_sim {
   uint64_t step = 0;
   std::vector<float> hr, hlr;
   std::vector<uint8_t> hfl;
-  bool grid_valid = false, lgrid_valid = false, dstep_dirty = true;
-  int clean_bs = 0;
+  bool lgrid_valid = false, dstep_dirty = true;
   uint32_t n_samples = 0;
 
   // device
   int dev = 0;
   hipStream_t stream = nullptr;
-  DevBuf<float4> r, er, pstar, spos, lr, lspos, aux;
-  DevBuf<uint32_t> cstart, lstart, count, bs0, bs1, bucket, soi, flags, cmap;
+  DevBuf<float4> r, er, pstar, lr, lspos, aux;
+  DevBuf<uint32_t> lstart, flags, cmap;
   DevBuf<uint64_t> dstep;
   DevBuf<unsigned long long> dcount;
-  hipGraphExec_t graphs[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  // one uniform grid: binning buffers + ordered cell list
+  struct HostGrid {
+    GridGeom g{};
+    DevBuf<uint32_t> count, bs0, bs1, bucket, cstart, soi;
+    DevBuf<float4> spos;
+    int clean = 0;       // which per-block total buffer is zero
+    bool valid = false;  // holds the cell list of the current positions
+    bool allocated = false;
+    uint32_t *bs(int k) { return k ? bs1.p : bs0.p; }
+    void alloc(size_t n_obj) {
+      count.alloc(g.n_c); bs0.alloc(g.nblk); bs1.alloc(g.nblk);
+      bucket.alloc((size_t)g.n_c * g.cap); cstart.alloc(g.n_c + 1); soi.alloc(n_obj); spos.alloc(n_obj);
+      hip_check(hipMemset(count.p, 0, (size_t)g.n_c * 4));
+      hip_check(hipMemset(bs0.p, 0, (size_t)g.nblk * 4));
+      hip_check(hipMemset(bs1.p, 0, (size_t)g.nblk * 4));
+      hip_check(hipMemset(cstart.p, 0, ((size_t)g.n_c + 1) * 4));
+      clean = 0;
+      valid = false;
+      allocated = true;
+    }
+    void free_() {
+      DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &bucket, &cstart, &soi};
+      for (auto *b : u) b->free_();
+      spos.free_();
+      allocated = false;
+    }
+  };
+  HostGrid gf; // fine grid of every bead
+  GridGeom lg{}; // LBS sites on the fine geometry (uses gf's binning buffers)
+  HostGrid gh; // coarse grid of het beads
+  HostGrid gx; // coarse grid of every bead in ccp's geometry (cell-list export)
+  hipGraphExec_t graphs[3][2][2] = {};
 
   // misc
   std::string err;
   ccp_log_fn log_fn = nullptr;
   void *log_user = nullptr;
 
-  uint32_t *bs(int k) { return k ? bs1.p : bs0.p; }
+  void drop_graphs() {
+    for (auto &gm : graphs)
+      for (auto &gr : gm)
+        for (auto &g : gr)
+          if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+          }
+  }
 
   ~ccp_sim() {
-    for (auto &gm : graphs)
-      for (auto &g : gm)
-        if (g) (void)hipGraphExecDestroy(g);
-    DevBuf<float4> *f4[] = {&r, &er, &pstar, &spos, &lr, &lspos, &aux};
+    drop_graphs();
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &lr, &lspos, &aux};
     for (auto *b : f4) b->free_();
-    DevBuf<uint32_t> *u4[] = {&cstart, &lstart, &count, &bs0, &bs1, &bucket, &soi, &flags, &cmap};
+    DevBuf<uint32_t> *u4[] = {&lstart, &flags, &cmap};
     for (auto *b : u4) b->free_();
+    gf.free_();
+    gh.free_();
+    gx.free_();
     dstep.free_();
     dcount.free_();
     if (stream) (void)hipStreamDestroy(stream);
@@ -249,10 +289,12 @@ struct ccp_sim {
     if (mode == MODE_ICG) k.sd = 0.0f;
     return k;
   }
-  void build_grid(const float4 *pos, int n_obj, float4 *out_spos, uint32_t *out_cstart, uint32_t *out_soi);
+  void build_grid(HostGrid &G, const GridGeom &geom, const float4 *pos, int n_obj, bool het_only, float4 *out_spos,
+                  uint32_t *out_cstart, uint32_t *out_soi);
   bool take_overflow();
   void grow_capacity();
   void ensure_grids();
+  void invalidate() { gf.valid = gh.valid = gx.valid = false; }
   void sync_step();
   void launch_stage(int mode, int stage);
   void launch_step(int mode) {
@@ -336,13 +378,36 @@ void ccp_sim::configure(const ParMap &par) {
   cutoff = std::max(std::max(cutoff, LCO), contact_rc);
   csl = par.get<float>("cell_size", lj ? rc_lj : ACO);
   if (!(csl >= cutoff)) throw Error(CCP_ERR_PARAM, "cell_size below the interaction cutoff");
-  cps = 2 * (uint32_t)std::ceil(d_m / csl);
-  if (cps < 4) cps = 4;
-  if ((uint64_t)cps * cps * cps >= (1ull << 31)) throw Error(CCP_ERR_PARAM, "cell_size too small for the nucleus");
-  n_c = cps * cps * cps;
-  nblk = (n_c + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  cap = par.get<uint32_t>("cell_capacity", 32u);
+  // fine grid: side >= every short-range cutoff (pair repulsion, LBS
+  // binding, the initial-overlap distance)
+  // (the split is off when fine_cell_size equals cell_size, the default)
+  const float short_cut = lj ? (float)std::pow(2.0, 1.0 / 6.0) : RCO;
+  csl_f = par.get<float>("fine_cell_size", csl);
+  if (!(csl_f >= short_cut && csl_f >= LCO && csl_f >= MIS && csl_f >= contact_rc / 2.0f))
+    throw Error(CCP_ERR_PARAM, "fine_cell_size below the short-range cutoff");
+  if (csl_f > csl) csl_f = csl;
+  split = csl_f < csl;
+  uint32_t cap = par.get<uint32_t>("cell_capacity", 32u);
+  uint32_t cap_f = par.get<uint32_t>("fine_cell_capacity", 16u);
   if (!(1 <= cap && cap <= 1024)) throw Error(CCP_ERR_PARAM, "cell_capacity out of range");
+  if (!(1 <= cap_f && cap_f <= 1024)) throw Error(CCP_ERR_PARAM, "fine_cell_capacity out of range");
+  auto geometry = [&](float side, uint32_t capacity) {
+    GridGeom g{};
+    uint32_t cps = 2 * (uint32_t)std::ceil(d_m / side);
+    if (cps < 4) cps = 4;
+    if ((uint64_t)cps * cps * cps >= (1ull << 31)) throw Error(CCP_ERR_PARAM, "cell_size too small for the nucleus");
+    g.csl = side;
+    g.cps = (int)cps;
+    g.half = (int)(cps / 2);
+    g.n_c = (int)(cps * cps * cps);
+    g.cap = (int)capacity;
+    g.nblk = (g.n_c + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    return g;
+  };
+  gh.g = geometry(csl, cap);
+  gx.g = gh.g;
+  gf.g = geometry(csl_f, split ? cap_f : cap);
+  lg = gf.g;
   fcap = par.get<float>("force_cap", lj ? 0.2f / dt : 0.0f);
   if (!(fcap >= 0.0f)) throw Error(CCP_ERR_PARAM, "force_cap out of range");
   graph_steps = par.get<uint32_t>("graph_steps", 64u);
@@ -403,12 +468,10 @@ void ccp_sim::fill_kparams() {
   KParams &k = kp;
   k.N = (int)N;
   k.n_l = (int)n_l;
-  k.cps = (int)cps;
-  k.half = (int)(cps / 2);
-  k.n_c = (int)n_c;
-  k.cap = (int)cap;
-  k.nblk = (int)nblk;
-  k.csl = csl;
+  k.gf = gf.g;
+  k.gh = gh.g;
+  if (!split) k.gh.n_c = 0;
+  k.split = split ? 1 : 0;
   k.R_n = R_n; k.R_o = R_o; k.R_b = R_b;
   k.noc = noc; k.boc = boc; k.nod = nod; k.bod = bod;
   const bool lj = ff == CCP_FF_LJ;
@@ -449,24 +512,18 @@ void ccp_sim::fill_kparams() {
 void ccp_sim::allocate() {
   hip_check(hipGetDevice(&dev));
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  r.alloc(N); er.alloc(N); pstar.alloc(N); spos.alloc(N); aux.alloc(N); soi.alloc(N);
-  lr.alloc(n_l); lspos.alloc(n_l);
-  cstart.alloc(n_c + 1); lstart.alloc(n_c + 1); count.alloc(n_c);
-  bs0.alloc(nblk); bs1.alloc(nblk);
-  bucket.alloc((size_t)n_c * cap);
+  r.alloc(N); er.alloc(N); pstar.alloc(N); aux.alloc(N);
+  lr.alloc(n_l); lspos.alloc(n_l); lstart.alloc(gf.g.n_c + 1);
   flags.alloc(FLAG_COUNT);
   cmap.alloc((size_t)cbins * cbins);
   dstep.alloc(1);
   dcount.alloc(1);
-  hip_check(hipMemset(count.p, 0, n_c * sizeof(uint32_t)));
-  hip_check(hipMemset(bs0.p, 0, nblk * sizeof(uint32_t)));
-  hip_check(hipMemset(bs1.p, 0, nblk * sizeof(uint32_t)));
+  gf.alloc(std::max(N, n_l));
+  if (split) gh.alloc(N);
   hip_check(hipMemset(flags.p, 0, FLAG_COUNT * sizeof(uint32_t)));
   hip_check(hipMemset(cmap.p, 0, (size_t)cbins * cbins * sizeof(uint32_t)));
   hip_check(hipMemset(dstep.p, 0, sizeof(uint64_t)));
-  hip_check(hipMemset(cstart.p, 0, (n_c + 1) * sizeof(uint32_t)));
-  hip_check(hipMemset(lstart.p, 0, (n_c + 1) * sizeof(uint32_t)));
-  clean_bs = 0;
+  hip_check(hipMemset(lstart.p, 0, (gf.g.n_c + 1) * sizeof(uint32_t)));
 }
 
 uint32_t ccp_sim::wbits(uint32_t i) const {
@@ -492,7 +549,7 @@ void ccp_sim::upload_positions() {
   }
   hip_check(hipMemcpyAsync(r.p, h.data(), N * sizeof(float4), hipMemcpyHostToDevice, stream));
   hip_check(hipStreamSynchronize(stream));
-  grid_valid = false;
+  invalidate();
 }
 
 void ccp_sim::upload_lbs() {
@@ -530,14 +587,15 @@ void ccp_sim::check_flags() {
   }
 }
 
-// Bin n_obj positions and order them into (out_spos, out_cstart[, out_soi]).
-void ccp_sim::build_grid(const float4 *pos, int n_obj, float4 *out_spos, uint32_t *out_cstart, uint32_t *out_soi) {
-  KParams k = kp;
-  launch_bin(stream, k, n_obj, pos, count.p, bs(clean_bs), bucket.p, flags.p);
-  launch_order(stream, k, n_obj, count.p, bs(clean_bs), bs(clean_bs ^ 1), bucket.p, pos, out_spos, out_soi, out_cstart,
-               flags.p, nullptr);
+// Bin n_obj positions with G's buffers in geometry `geom` and order them
+// into (out_spos, out_cstart[, out_soi]).
+void ccp_sim::build_grid(HostGrid &G, const GridGeom &geom, const float4 *pos, int n_obj, bool het_only,
+                         float4 *out_spos, uint32_t *out_cstart, uint32_t *out_soi) {
+  launch_bin(stream, kp, geom, n_obj, pos, het_only, G.count.p, G.bs(G.clean), G.bucket.p, flags.p);
+  OrderJob j{geom, G.count.p, G.bs(G.clean), G.bs(G.clean ^ 1), G.bucket.p, pos, out_spos, out_soi, out_cstart};
+  launch_order(stream, j, nullptr, flags.p, nullptr);
   hip_check(hipGetLastError());
-  clean_bs ^= 1;
+  G.clean ^= 1;
 }
 
 // Overflow flag of the last builds (synchronizes the stream); clears it.
@@ -549,35 +607,42 @@ bool ccp_sim::take_overflow() {
   return f != 0;
 }
 
-// Double the cell capacity (bucket storage); captured graphs refer to the
-// old buffer and are dropped.
+// Double every grid's cell capacity (bucket storage); captured graphs refer
+// to the old buffers and are dropped.
 void ccp_sim::grow_capacity() {
-  if (cap >= 4096) throw Error(CCP_ERR_STATE, "cell_capacity exceeded; raise cell_capacity");
+  if (gh.g.cap >= 4096 || gf.g.cap >= 4096)
+    throw Error(CCP_ERR_STATE, "cell_capacity exceeded; raise cell_capacity");
   hip_check(hipStreamSynchronize(stream));
-  for (auto &gm : graphs)
-    for (auto &g : gm)
-      if (g) {
-        hip_check(hipGraphExecDestroy(g));
-        g = nullptr;
-      }
-  cap *= 2;
-  kp.cap = (int)cap;
-  bucket.free_();
-  bucket.alloc((size_t)n_c * cap);
-  grid_valid = lgrid_valid = false;
+  drop_graphs();
+  for (HostGrid *G : {&gf, &gh, &gx}) {
+    G->g.cap *= 2;
+    if (G->allocated) {
+      G->bucket.free_();
+      G->bucket.alloc((size_t)G->g.n_c * G->g.cap);
+    }
+    G->valid = false;
+  }
+  lg.cap = gf.g.cap;
+  kp.gf = gf.g;
+  kp.gh = gh.g;
+  if (!split) kp.gh.n_c = 0;
+  lgrid_valid = false;
 }
 
-// Cell lists of the LBS sites and of the current positions; grows the
-// capacity until every cell fits (not used inside the step loop).
+// Cell lists of the LBS sites and of the current positions (fine grid of
+// every bead, coarse grid of het beads); grows the capacity until every
+// cell fits (not used inside the step loop).
 void ccp_sim::ensure_grids() {
-  if (grid_valid && lgrid_valid) return;
+  if (gf.valid && (gh.valid || !split) && lgrid_valid) return;
   for (;;) {
-    if (!lgrid_valid) build_grid(lr.p, (int)n_l, lspos.p, lstart.p, nullptr);
-    if (!grid_valid) build_grid(r.p, (int)N, spos.p, cstart.p, nullptr);
+    if (!lgrid_valid) build_grid(gf, lg, lr.p, (int)n_l, false, lspos.p, lstart.p, nullptr);
+    if (!gf.valid) build_grid(gf, gf.g, r.p, (int)N, false, gf.spos.p, gf.cstart.p, nullptr);
+    if (split && !gh.valid) build_grid(gh, gh.g, r.p, (int)N, true, gh.spos.p, gh.cstart.p, nullptr);
     if (!take_overflow()) break;
     grow_capacity();
   }
-  grid_valid = lgrid_valid = true;
+  gf.valid = lgrid_valid = true;
+  gh.valid = split;
 }
 
 void ccp_sim::sync_step() {
@@ -590,23 +655,36 @@ void ccp_sim::sync_step() {
 
 void ccp_sim::launch_stage(int mode, int stage) {
   KParams k = mode_params(mode);
+  const bool het = split && mode != MODE_ICG;
+  if (!het) k.gh.n_c = 0; // no het grid (single grid, or the soft relaxation)
   KPtrs q{};
-  q.spos = spos.p;
-  q.cstart = cstart.p;
+  q.fspos = gf.spos.p;
+  q.fcstart = gf.cstart.p;
+  q.hspos = gh.spos.p;
+  q.hcstart = gh.cstart.p;
   q.lspos = lspos.p;
   q.lstart = lstart.p;
   q.pos = stage == 1 ? r.p : er.p;
   q.out0 = stage == 1 ? er.p : r.p;
   q.pstar = pstar.p;
-  q.count = count.p;
-  q.blocksum = bs(clean_bs);
-  q.bucket = bucket.p;
+  q.fcount = gf.count.p;
+  q.fblocksum = gf.bs(gf.clean);
+  q.fbucket = gf.bucket.p;
+  q.hcount = gh.count.p;
+  q.hblocksum = gh.bs(gh.clean);
+  q.hbucket = gh.bucket.p;
   q.flags = flags.p;
   q.step = dstep.p;
   launch_force(stream, mode, stage, k, q);
-  launch_order(stream, k, (int)N, count.p, bs(clean_bs), bs(clean_bs ^ 1), bucket.p, q.out0, spos.p, nullptr,
-               cstart.p, flags.p, stage == 2 ? dstep.p : nullptr);
-  clean_bs ^= 1;
+  OrderJob jf{gf.g, gf.count.p, gf.bs(gf.clean), gf.bs(gf.clean ^ 1), gf.bucket.p, q.out0, gf.spos.p, nullptr,
+              gf.cstart.p};
+  OrderJob jh{gh.g, gh.count.p, gh.bs(gh.clean), gh.bs(gh.clean ^ 1), gh.bucket.p, q.out0, gh.spos.p, nullptr,
+              gh.cstart.p};
+  launch_order(stream, jf, het ? &jh : nullptr, flags.p, stage == 2 ? dstep.p : nullptr);
+  gf.clean ^= 1;
+  if (het) gh.clean ^= 1;
+  else gh.valid = false;
+  gx.valid = false;
 }
 
 // n steps without contact sampling: whole graphs of graph_steps, then eager.
@@ -614,23 +692,25 @@ void ccp_sim::run_chunk(uint64_t n, int mode) {
   if (n == 0) return;
   const uint64_t G = graph_steps;
   if (n >= G) {
-    hipGraphExec_t &ge = graphs[mode][clean_bs];
+    hipGraphExec_t &ge = graphs[mode][gf.clean][gh.clean];
+    const int cf = gf.clean, ch = gh.clean;
+    const bool vh = gh.valid;
     if (!ge) {
-      const int cb = clean_bs;
       hipGraph_t g;
       hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       for (uint64_t s = 0; s < G; ++s) launch_step(mode);
       hip_check(hipStreamEndCapture(stream, &g));
       hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
       hip_check(hipGraphDestroy(g));
-      clean_bs = cb; // capture recorded the launches; replays do the work
     }
-    const int cb0 = clean_bs;
     while (n >= G) {
       hip_check(hipGraphLaunch(ge, stream));
       n -= G;
     }
-    clean_bs = cb0; // each step flips the clean buffer twice
+    // each step flips the clean buffers twice
+    gf.clean = cf;
+    gh.clean = ch;
+    gh.valid = mode != MODE_ICG ? vh : false;
   }
   for (uint64_t s = 0; s < n; ++s) launch_step(mode);
   hip_check(hipGetLastError());
@@ -646,7 +726,7 @@ void ccp_sim::run_steps(uint64_t n, int mode) {
     step += chunk;
     n -= chunk;
     if (cinterval && mode != MODE_ICG && step % cinterval == 0) {
-      launch_contacts(stream, kp, spos.p, cstart.p, cmap.p);
+      launch_contacts(stream, kp, gf.spos.p, gf.cstart.p, cmap.p);
       ++n_samples;
     }
   }
@@ -852,7 +932,7 @@ void ccp_sim::set_particle_positions(HostRng &g) {
 uint64_t ccp_sim::count_overlaps(float dmin) {
   ensure_grids();
   hip_check(hipMemsetAsync(dcount.p, 0, sizeof(unsigned long long), stream));
-  launch_overlaps(stream, kp, spos.p, cstart.p, dmin * dmin, dcount.p);
+  launch_overlaps(stream, kp, gf.spos.p, gf.cstart.p, dmin * dmin, dcount.p);
   unsigned long long c = 0;
   hip_check(hipMemcpyAsync(&c, dcount.p, sizeof(c), hipMemcpyDeviceToHost, stream));
   hip_check(hipStreamSynchronize(stream));
@@ -965,9 +1045,11 @@ int ccp_get_info(const ccp_sim *h, ccp_info *o) {
   o->n_l = h->n_l;
   o->n_chains = (uint32_t)h->chain_off.size() - 1;
   o->force_field = (uint32_t)h->ff;
-  o->cps = h->cps;
-  o->n_c = h->n_c;
-  o->cell_capacity = h->cap;
+  o->cps = (uint32_t)h->gx.g.cps;
+  o->n_c = (uint32_t)h->gx.g.n_c;
+  o->cell_capacity = (uint32_t)h->gh.g.cap;
+  o->fine_cps = (uint32_t)h->gf.g.cps;
+  o->fine_csl = h->gf.g.csl;
   o->spf = h->spf;
   o->fpf = h->fpf;
   o->contact_bins = h->cbins;
@@ -1284,8 +1366,10 @@ int ccp_compute_forces(ccp_sim *h, int ff, float *f_xyz, size_t n) {
     h->ensure_grids();
     KParams k = h->mode_params(ff);
     KPtrs q{};
-    q.spos = h->spos.p;
-    q.cstart = h->cstart.p;
+    q.fspos = h->gf.spos.p;
+    q.fcstart = h->gf.cstart.p;
+    q.hspos = h->gh.spos.p;
+    q.hcstart = h->gh.cstart.p;
     q.lspos = h->lspos.p;
     q.lstart = h->lstart.p;
     q.pos = h->r.p;
@@ -1307,24 +1391,26 @@ int ccp_compute_forces(ccp_sim *h, int ff, float *f_xyz, size_t n) {
 
 int ccp_cell_list(ccp_sim *h, uint32_t *sci, uint32_t *soi, uint32_t *beg, uint32_t *end, size_t n, size_t n_c) {
   return guarded(h, [&] {
-    need(n == h->N && n_c == h->n_c && sci && soi && beg && end, "cell list size mismatch");
+    auto &G = h->gx;
+    need(n == h->N && n_c == (size_t)G.g.n_c && sci && soi && beg && end, "cell list size mismatch");
+    if (!G.allocated) G.alloc(h->N);
     for (;;) {
-      h->build_grid(h->r.p, (int)h->N, h->spos.p, h->cstart.p, h->soi.p);
+      h->build_grid(G, G.g, h->r.p, (int)h->N, false, G.spos.p, G.cstart.p, G.soi.p);
       if (!h->take_overflow()) break;
       h->grow_capacity();
     }
-    h->grid_valid = true;
-    std::vector<uint32_t> cs(h->n_c + 1);
-    hip_check(hipMemcpyAsync(soi, h->soi.p, 4 * n, hipMemcpyDeviceToHost, h->stream));
-    hip_check(hipMemcpyAsync(cs.data(), h->cstart.p, 4 * (n_c + 1), hipMemcpyDeviceToHost, h->stream));
+    G.valid = true;
+    std::vector<uint32_t> cs((size_t)G.g.n_c + 1);
+    hip_check(hipMemcpyAsync(soi, G.soi.p, 4 * n, hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipMemcpyAsync(cs.data(), G.cstart.p, 4 * (n_c + 1), hipMemcpyDeviceToHost, h->stream));
     hip_check(hipStreamSynchronize(h->stream));
     h->check_flags();
     // ccp layout: beg/end of non-empty cells, 0/0 for empty ones (sugrid.cu:38-77)
-    for (uint32_t c = 0; c < h->n_c; ++c) {
+    for (size_t c = 0; c < n_c; ++c) {
       const bool full = cs[c + 1] > cs[c];
       beg[c] = full ? cs[c] : 0u;
       end[c] = full ? cs[c + 1] : 0u;
-      for (uint32_t s = cs[c]; s < cs[c + 1]; ++s) sci[s] = c;
+      for (uint32_t s = cs[c]; s < cs[c + 1]; ++s) sci[s] = (uint32_t)c;
     }
   });
 }
@@ -1349,7 +1435,7 @@ int ccp_noise(ccp_sim *h, uint64_t step, float *z_xyz, uint32_t *attempts, size_
 int ccp_count_overlaps(ccp_sim *h, float dmin, uint64_t *count) {
   return guarded(h, [&] {
     need(count != nullptr, "null count");
-    need(dmin > 0.0f && dmin <= h->csl, "dmin must be in (0, cell_size]");
+    need(dmin > 0.0f && dmin <= h->gf.g.csl, "dmin must be in (0, fine_cell_size]");
     *count = h->count_overlaps(dmin);
   });
 }
@@ -1357,7 +1443,7 @@ int ccp_count_overlaps(ccp_sim *h, float dmin, uint64_t *count) {
 int ccp_contact_sample(ccp_sim *h) {
   return guarded(h, [&] {
     h->ensure_grids();
-    launch_contacts(h->stream, h->kp, h->spos.p, h->cstart.p, h->cmap.p);
+    launch_contacts(h->stream, h->kp, h->gf.spos.p, h->gf.cstart.p, h->cmap.p);
     hip_check(hipGetLastError());
     ++h->n_samples;
   });
@@ -1390,29 +1476,38 @@ int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]) {
     for (auto &e : ev) hip_check(hipEventCreate(&e));
     double acc[4] = {0, 0, 0, 0};
     for (uint32_t s = 0; s < n_steps; ++s) {
-      // launch_stage = force launch + order launch; time them separately
       for (int stage = 1; stage <= 2; ++stage) {
+        // launch_stage = force launch + order launch; bracket each with events
         KParams k = h->mode_params(mode);
         KPtrs q{};
-        q.spos = h->spos.p;
-        q.cstart = h->cstart.p;
+        q.fspos = h->gf.spos.p;
+        q.fcstart = h->gf.cstart.p;
+        q.hspos = h->gh.spos.p;
+        q.hcstart = h->gh.cstart.p;
         q.lspos = h->lspos.p;
         q.lstart = h->lstart.p;
         q.pos = stage == 1 ? h->r.p : h->er.p;
         q.out0 = stage == 1 ? h->er.p : h->r.p;
         q.pstar = h->pstar.p;
-        q.count = h->count.p;
-        q.blocksum = h->bs(h->clean_bs);
-        q.bucket = h->bucket.p;
+        q.fcount = h->gf.count.p;
+        q.fblocksum = h->gf.bs(h->gf.clean);
+        q.fbucket = h->gf.bucket.p;
+        q.hcount = h->gh.count.p;
+        q.hblocksum = h->gh.bs(h->gh.clean);
+        q.hbucket = h->gh.bucket.p;
         q.flags = h->flags.p;
         q.step = h->dstep.p;
         hip_check(hipEventRecord(ev[0], h->stream));
         launch_force(h->stream, mode, stage, k, q);
         hip_check(hipEventRecord(ev[1], h->stream));
-        launch_order(h->stream, k, (int)h->N, h->count.p, h->bs(h->clean_bs), h->bs(h->clean_bs ^ 1), h->bucket.p,
-                     q.out0, h->spos.p, nullptr, h->cstart.p, h->flags.p, stage == 2 ? h->dstep.p : nullptr);
+        OrderJob jf{h->gf.g, h->gf.count.p, h->gf.bs(h->gf.clean), h->gf.bs(h->gf.clean ^ 1), h->gf.bucket.p,
+                    q.out0, h->gf.spos.p, nullptr, h->gf.cstart.p};
+        OrderJob jh{h->gh.g, h->gh.count.p, h->gh.bs(h->gh.clean), h->gh.bs(h->gh.clean ^ 1), h->gh.bucket.p,
+                    q.out0, h->gh.spos.p, nullptr, h->gh.cstart.p};
+        launch_order(h->stream, jf, h->split ? &jh : nullptr, h->flags.p, stage == 2 ? h->dstep.p : nullptr);
         hip_check(hipEventRecord(ev[2], h->stream));
-        h->clean_bs ^= 1;
+        h->gf.clean ^= 1;
+        if (h->split) h->gh.clean ^= 1;
         hip_check(hipEventSynchronize(ev[2]));
         float a, b;
         hip_check(hipEventElapsedTime(&a, ev[0], ev[1]));

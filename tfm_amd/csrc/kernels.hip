@@ -1,20 +1,27 @@
 // gfx950 kernels of the ccp Brownian-dynamics step.
 //
 // One stochastic-Heun stage is two launches:
-//   k_force<MODE, STAGE>  one thread per bead in cell-sorted order: pair
-//                         forces from the 9 contiguous neighbour rows of the
-//                         sorted grid, LBS binding, bonds/bending from the
-//                         bead-ordered array, wall force, the Heun update,
-//                         and binning of the updated position into the next
-//                         cell list (wave-aggregated atomics on per-cell
-//                         counters and per-scan-block totals, bucket slot).
-//   k_order               one thread per cell: exclusive scan of the counts
-//                         (block prefix from the per-block totals), then a
-//                         rank-by-bead-index ordering of the cell's bucket, so
-//                         the sorted arrays equal a stable sort by cell (the
-//                         result of ccp's cub::DeviceRadixSort::SortPairs over
-//                         identity values, sugrid.cu:125-133) bit for bit, and
-//                         a gather of the positions into sorted order.
+//   k_force<MODE, STAGE>  one thread per bead in fine-grid order. Short-range
+//                         pairs (every pair type, cutoff <= fine cell side)
+//                         from the 9 contiguous neighbour rows of the fine
+//                         grid; for het beads the long-range het-het pairs
+//                         from the coarse grid that holds het beads only; LBS
+//                         binding, bonds/bending from the bead-ordered array,
+//                         the wall, the Heun update, and binning of the new
+//                         position into the next fine and het grids
+//                         (wave-aggregated atomics on per-cell counters and
+//                         per-scan-block totals, bucket slot).
+//   k_order               one thread per cell of both grids: exclusive scan
+//                         of the counts (block prefix from the per-block
+//                         totals), then a rank-by-bead-index ordering of each
+//                         cell's bucket, so the ordered arrays equal a stable
+//                         sort by cell (the result of ccp's
+//                         cub::DeviceRadixSort::SortPairs over identity values,
+//                         sugrid.cu:125-133) bit for bit, and a gather of the
+//                         positions into cell order.
+// Splitting the pair sum by cutoff keeps the result of ccp's single 27-cell
+// scan (calc_srf, chrsim.cu:277-317) while a short-range candidate costs a
+// 1.25-sigma cell instead of a 2-2.5-sigma one.
 // Reference behaviour followed: chrsim.cu:13-416, sugrid.cu:15-133.
 
 #include "kernels.hpp"
@@ -121,46 +128,77 @@ __device__ __forceinline__ void agg_add_noret(uint32_t *base, uint32_t key) {
 // ---------------------------------------------------------------- grid
 // Cell coordinate along one axis: floor(x / csl) + cps/2 as ccp's
 // calc_indexes (sugrid.cu:26-34), clamped into the grid (ccp has no clamp).
-__device__ __forceinline__ int cell_coord(float x, const KParams &p, bool &outside) {
-  int g = (int)floorf(x / p.csl) + p.half;
-  if (g < 0) { g = 0; outside = true; }
-  if (g >= p.cps) { g = p.cps - 1; outside = true; }
-  return g;
+__device__ __forceinline__ int cell_coord(float x, const GridGeom &g, bool &outside) {
+  int c = (int)floorf(x / g.csl) + g.half;
+  if (c < 0) { c = 0; outside = true; }
+  if (c >= g.cps) { c = g.cps - 1; outside = true; }
+  return c;
 }
 
-__device__ __forceinline__ void bin_position(const KParams &p, const KPtrs &q, uint32_t i, float x, float y, float z) {
+__device__ __forceinline__ void bin_into(const GridGeom &g, uint32_t *count, uint32_t *blocksum, uint32_t *bucket,
+                                         uint32_t *flags, uint32_t i, float x, float y, float z) {
   bool outside = false;
-  const int gx = cell_coord(x, p, outside), gy = cell_coord(y, p, outside), gz = cell_coord(z, p, outside);
-  const uint32_t key = (uint32_t)((gz * p.cps + gy) * p.cps + gx);
-  const uint32_t slot = agg_add(q.count, key);
-  if (slot < (uint32_t)p.cap) q.bucket[(size_t)key * p.cap + slot] = i;
-  agg_add_noret(q.blocksum, key / SCAN_BLOCK);
-  if (outside) atomicAdd(q.flags + FLAG_OUTSIDE, 1u);
+  const int gx = cell_coord(x, g, outside), gy = cell_coord(y, g, outside), gz = cell_coord(z, g, outside);
+  const uint32_t key = (uint32_t)((gz * g.cps + gy) * g.cps + gx);
+  const uint32_t slot = agg_add(count, key);
+  if (slot < (uint32_t)g.cap) bucket[(size_t)key * g.cap + slot] = i;
+  agg_add_noret(blocksum, key / SCAN_BLOCK);
+  if (outside) atomicAdd(flags + FLAG_OUTSIDE, 1u);
+}
+
+// Bin a new bead position into the next fine grid and, for het beads, the
+// next het grid (the het-grid atomics run on the het lanes only).
+__device__ __forceinline__ void bin_bead(const KParams &p, const KPtrs &q, uint32_t i, uint32_t w, float x, float y,
+                                         float z) {
+  bin_into(p.gf, q.fcount, q.fblocksum, q.fbucket, q.flags, i, x, y, z);
+  if (p.gh.n_c > 0 && (w & W_HET)) bin_into(p.gh, q.hcount, q.hblocksum, q.hbucket, q.flags, i, x, y, z);
   if (!(isfinite(x) && isfinite(y) && isfinite(z))) atomicAdd(q.flags + FLAG_NONFINITE, 1u);
 }
 
+// Slot ranges of the 9 neighbour rows (x-1..x+1 of rows y+dy, z+dz) of the
+// cell holding (x, y, z); rows outside the grid get an empty range. All 18
+// start lookups are issued together.
+struct Rows {
+  uint32_t lo[9], hi[9];
+  int c0[9], c1[9];
+};
+
+__device__ __forceinline__ void neighbour_rows(const GridGeom &g, const uint32_t *__restrict__ cstart, float x,
+                                               float y, float z, Rows &R) {
+  bool outside = false;
+  const int cx = cell_coord(x, g, outside), cy = cell_coord(y, g, outside), cz = cell_coord(z, g, outside);
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int zz = cz + r / 3 - 1, yy = cy + r % 3 - 1;
+    const bool ok = zz >= 0 && zz < g.cps && yy >= 0 && yy < g.cps;
+    const int row = (zz * g.cps + yy) * g.cps;
+    R.c0[r] = ok ? row + max(cx - 1, 0) : 0;
+    R.c1[r] = ok ? row + min(cx + 1, g.cps - 1) + 1 : 0;
+    R.lo[r] = cstart[R.c0[r]];
+    R.hi[r] = cstart[R.c1[r]];
+  }
+}
+
 // ---------------------------------------------------------------- forces
-// Non-bonded pair force on bead i from the sorted entry o, accumulated into f.
+// Short-range pair force on bead i from entry o (every pair type), with
+// |i - j| <= 1 skipped (calc_cell_srf, chrsim.cu:250).
 // CCP: Wang-Frenkel e_p (18 d^4 - 96 d^2 + 96)/d^8 (calc_ppf<DST>,
-// chrsim.cu:173-185) cut at aco for het-het pairs and at rco otherwise.
-// ICG: soft repulsion 128 (3 rco - 3 d) (calc_ppf<ICG>, :188-198).
-// LJ: WCA for d < 2^(1/6) sigma, LJ tail eps_het between het beads to rc_lj.
-// Pairs with |i - j| <= 1 are skipped (calc_cell_srf, :250).
+// chrsim.cu:173-185) for d <= rco, the cutoff every pair type has.
+// ICG: soft repulsion 128 (3 rco - 3 d) for d <= rco (calc_ppf<ICG>, :188-198).
+// LJ: WCA for d < 2^(1/6) sigma.
 template <int MODE>
-__device__ __forceinline__ void pair_force(const KParams &p, float rx, float ry, float rz, uint32_t wi, float4 o,
+__device__ __forceinline__ void pair_short(const KParams &p, float rx, float ry, float rz, uint32_t wi, float4 o,
                                            float &fx, float &fy, float &fz) {
   const uint32_t wj = wbits(o);
   const uint32_t i = wi & ID_MASK, j = wj & ID_MASK;
   const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
   const float d2 = dx * dx + dy * dy + dz * dz;
   const bool nb = (j > i + 1u) || (i > j + 1u);
-  const bool hhi = (wi & wj & W_HET) != 0u;
   float fac;
   if constexpr (MODE == MODE_CCP) {
     const float u = fast_rcp(d2);
-    const float rc2 = hhi ? p.aco2 : p.rco2;
     fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
-    fac = (nb && d2 <= rc2) ? fac : 0.0f;
+    fac = (nb && d2 <= p.rco2) ? fac : 0.0f;
   } else if constexpr (MODE == MODE_ICG) {
     const float d = sqrtf(d2);
     fac = 128.0f * (3.0f * p.rco - 3.0f * d);
@@ -169,10 +207,35 @@ __device__ __forceinline__ void pair_force(const KParams &p, float rx, float ry,
     const float u = fast_rcp(d2);
     const float s2 = p.sig2 * u;
     const float s6 = s2 * s2 * s2;
-    const float base = 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
-    const bool wca = nb && d2 < p.rm2;
-    const bool tail = nb && hhi && !wca && d2 < p.rc_lj2;
-    fac = wca ? p.eps_wca * base : (tail ? p.eps_het * base : 0.0f);
+    fac = p.eps_wca * 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
+    fac = (nb && d2 < p.rm2) ? fac : 0.0f;
+  }
+  fx += fac * dx;
+  fy += fac * dy;
+  fz += fac * dz;
+}
+
+// Long-range het-het pair force (both beads het by construction).
+// CCP: the Wang-Frenkel force of A-A pairs for rco < d <= aco.
+// LJ: the LJ tail eps_het for 2^(1/6) sigma <= d < rc_lj.
+template <int MODE>
+__device__ __forceinline__ void pair_tail(const KParams &p, float rx, float ry, float rz, uint32_t wi, float4 o,
+                                          float &fx, float &fy, float &fz) {
+  const uint32_t wj = wbits(o);
+  const uint32_t i = wi & ID_MASK, j = wj & ID_MASK;
+  const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  const bool nb = (j > i + 1u) || (i > j + 1u);
+  const float u = fast_rcp(d2);
+  float fac;
+  if constexpr (MODE == MODE_CCP) {
+    fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
+    fac = (nb && d2 > p.rco2 && d2 <= p.aco2) ? fac : 0.0f;
+  } else {
+    const float s2 = p.sig2 * u;
+    const float s6 = s2 * s2 * s2;
+    fac = p.eps_het * 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
+    fac = (nb && d2 >= p.rm2 && d2 < p.rc_lj2) ? fac : 0.0f;
   }
   fx += fac * dx;
   fy += fac * dy;
@@ -318,66 +381,131 @@ __device__ __forceinline__ void wall_force(const KParams &p, float rx, float ry,
 }
 
 // ---------------------------------------------------------------- kernels
+// Complete force of a non-bonded pair (|i - j| > 1): ccp's calc_ppf<DST>
+// (Wang-Frenkel, cut at aco for het-het pairs and at rco otherwise,
+// chrsim.cu:173-185), calc_ppf<ICG> (soft repulsion within rco, :188-198),
+// or WCA plus the het-het LJ tail.
+template <int MODE>
+__device__ __forceinline__ void pair_full(const KParams &p, float rx, float ry, float rz, uint32_t wi, float4 o,
+                                          float &fx, float &fy, float &fz) {
+  const uint32_t wj = wbits(o);
+  const uint32_t i = wi & ID_MASK, j = wj & ID_MASK;
+  const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  const bool nb = (j > i + 1u) || (i > j + 1u);
+  const bool hhi = (wi & wj & W_HET) != 0u;
+  float fac;
+  if constexpr (MODE == MODE_CCP) {
+    const float u = fast_rcp(d2);
+    fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
+    fac = (nb && d2 <= (hhi ? p.aco2 : p.rco2)) ? fac : 0.0f;
+  } else if constexpr (MODE == MODE_ICG) {
+    const float d = sqrtf(d2);
+    fac = 128.0f * (3.0f * p.rco - 3.0f * d);
+    fac = (nb && d <= p.rco) ? fac : 0.0f;
+  } else {
+    const float u = fast_rcp(d2);
+    const float s2 = p.sig2 * u;
+    const float s6 = s2 * s2 * s2;
+    const float base = 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
+    const bool wca = nb && d2 < p.rm2;
+    const bool tail = nb && hhi && !wca && d2 < p.rc_lj2;
+    fac = wca ? p.eps_wca * base : (tail ? p.eps_het * base : 0.0f);
+  }
+  fx += fac * dx;
+  fy += fac * dy;
+  fz += fac * dz;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2. Renumber them so XCD x runs one contiguous run of tiles: its beads,
+// and the neighbour rows they read, then stay in that XCD's L2.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b % N_XCD, k = b / N_XCD, q = nb / N_XCD, rem = nb % N_XCD;
+  return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + k;
+}
+
+// Candidates of the 9 rows, four at a time (independent loads in flight).
+// PASS 0: every pair on the all-bead grid (full cutoffs, or the short
+// cutoff when split); PASS 1: het-het tail on the het grid.
+template <int MODE, int PASS, bool SPLIT>
+__device__ __forceinline__ void scan_pass(const KParams &p, const float4 *__restrict__ spos, const Rows &R, float rx,
+                                          float ry, float rz, uint32_t w, float &fx, float &fy, float &fz) {
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    uint32_t k = R.lo[r];
+    const uint32_t h = R.hi[r];
+    for (; k + 4u <= h; k += 4u) {
+      const float4 a = spos[k], b = spos[k + 1], c = spos[k + 2], d = spos[k + 3];
+      if constexpr (PASS == 1) {
+        pair_tail<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
+        pair_tail<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
+        pair_tail<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
+        pair_tail<MODE>(p, rx, ry, rz, w, d, fx, fy, fz);
+      } else if constexpr (SPLIT) {
+        pair_short<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
+        pair_short<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
+        pair_short<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
+        pair_short<MODE>(p, rx, ry, rz, w, d, fx, fy, fz);
+      } else {
+        pair_full<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
+        pair_full<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
+        pair_full<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
+        pair_full<MODE>(p, rx, ry, rz, w, d, fx, fy, fz);
+      }
+    }
+    for (; k < h; ++k) {
+      if constexpr (PASS == 1) pair_tail<MODE>(p, rx, ry, rz, w, spos[k], fx, fy, fz);
+      else if constexpr (SPLIT) pair_short<MODE>(p, rx, ry, rz, w, spos[k], fx, fy, fz);
+      else pair_full<MODE>(p, rx, ry, rz, w, spos[k], fx, fy, fz);
+    }
+  }
+}
+
+// Force evaluation of one stage: one thread per bead in grid order,
+// candidates from the 9 contiguous neighbour rows of the all-bead grid
+// (and, when split, het-het tails from the het grid).
 // STAGE 0: write F(pos) by bead index. STAGE 1: Heun predictor
 // (exec_RK_1, chrsim.cu:364-388) er = r + f dt + rn, keeps
 // r + f dt/2 + rn. STAGE 2: corrector (exec_RK_2, :391-416)
 // r = (r + f dt/2 + rn) + ef dt/2. Stages 1 and 2 bin their output.
-template <int MODE, int STAGE>
+template <int MODE, int STAGE, bool SPLIT>
 __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
-  const int t = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  const int t = xcd_block() * BEAD_BLOCK + threadIdx.x;
   if (t >= p.N) return;
-  const float4 me = q.spos[t];
+  const float4 me = q.fspos[t];
   const uint32_t w = wbits(me);
   const uint32_t i = w & ID_MASK;
   const float rx = me.x, ry = me.y, rz = me.z;
-  bool outside = false;
-  const int cx = cell_coord(rx, p, outside), cy = cell_coord(ry, p, outside), cz = cell_coord(rz, p, outside);
 
-  // slot ranges of the 9 neighbour rows (x-1..x+1 of rows y+dy, z+dz), all
-  // loaded up front so the 18 start lookups are in flight together
-  uint32_t lo[9], hi[9];
-  int rc0[9], rc1[9];
-#pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
-    const bool ok = z >= 0 && z < p.cps && y >= 0 && y < p.cps;
-    const int row = (z * p.cps + y) * p.cps;
-    rc0[r] = ok ? row + max(cx - 1, 0) : 0;
-    rc1[r] = ok ? row + min(cx + 1, p.cps - 1) + 1 : 0;
-    lo[r] = q.cstart[rc0[r]];
-    hi[r] = q.cstart[rc1[r]];
-  }
+  Rows R;
+  neighbour_rows(p.gf, q.fcstart, rx, ry, rz, R);
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   bonded_force(p, q.pos, i, w, rx, ry, rz, fx, fy, fz);
   wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
-
-  // candidates four at a time: independent loads per iteration
-#pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    uint32_t k = lo[r];
-    const uint32_t h = hi[r];
-    for (; k + 4u <= h; k += 4u) {
-      const float4 a = q.spos[k], b = q.spos[k + 1], c = q.spos[k + 2], d = q.spos[k + 3];
-      pair_force<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
-      pair_force<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
-      pair_force<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
-      pair_force<MODE>(p, rx, ry, rz, w, d, fx, fy, fz);
-    }
-    for (; k < h; ++k) pair_force<MODE>(p, rx, ry, rz, w, q.spos[k], fx, fy, fz);
-  }
+  scan_pass<MODE, 0, SPLIT>(p, q.fspos, R, rx, ry, rz, w, fx, fy, fz);
   if (MODE != MODE_ICG && p.n_l > 0 && (w & W_LAD)) {
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-      const uint32_t llo = q.lstart[rc0[r]], lhi = q.lstart[rc1[r]];
+      const uint32_t llo = q.lstart[R.c0[r]], lhi = q.lstart[R.c1[r]];
       for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
+    }
+  }
+  if constexpr (SPLIT && MODE != MODE_ICG) {
+    if (w & W_HET) {
+      Rows H;
+      neighbour_rows(p.gh, q.hcstart, rx, ry, rz, H);
+      scan_pass<MODE, 1, true>(p, q.hspos, H, rx, ry, rz, w, fx, fy, fz);
     }
   }
   if (STAGE != 0 && p.fcap2 > 0.0f) {
     // optional safeguard of the WCA/LJ set: |F| <= fcap
     const float m2 = fx * fx + fy * fy + fz * fz;
     if (m2 > p.fcap2) {
-      const float s = sqrtf(p.fcap2 / m2);
-      fx *= s; fy *= s; fz *= s;
+      const float sc = sqrtf(p.fcap2 / m2);
+      fx *= sc; fy *= sc; fz *= sc;
     }
   }
 
@@ -393,49 +521,56 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
     const float ex = rx + fx * p.dt + nx, ey = ry + fy * p.dt + ny, ez = rz + fz * p.dt + nz;
     q.out0[i] = make_float4(ex, ey, ez, me.w);
     q.pstar[i] = make_float4(rx + fx * p.half_dt + nx, ry + fy * p.half_dt + ny, rz + fz * p.half_dt + nz, 0.0f);
-    bin_position(p, q, i, ex, ey, ez);
+    bin_bead(p, q, i, w, ex, ey, ez);
   } else {
     const float4 ps = q.pstar[i];
     const float nx = ps.x + fx * p.half_dt, ny = ps.y + fy * p.half_dt, nz = ps.z + fz * p.half_dt;
     q.out0[i] = make_float4(nx, ny, nz, me.w);
-    bin_position(p, q, i, nx, ny, nz);
+    bin_bead(p, q, i, w, nx, ny, nz);
   }
 }
 
-// Binning of an index-ordered array (initial build and LBS grid).
-__global__ __launch_bounds__(BEAD_BLOCK) void k_bin(KParams p, int n_obj, const float4 *pos, KPtrs q) {
+// Binning of an index-ordered array (initial builds, LBS grid, export grid).
+__global__ __launch_bounds__(BEAD_BLOCK) void k_bin(GridGeom g, int n_obj, const float4 *pos, int het_only,
+                                                    uint32_t *count, uint32_t *blocksum, uint32_t *bucket,
+                                                    uint32_t *flags) {
   const int i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
   if (i >= n_obj) return;
   const float4 v = pos[i];
-  bin_position(p, q, (uint32_t)i, v.x, v.y, v.z);
+  if (het_only && !(wbits(v) & W_HET)) return;
+  bin_into(g, count, blocksum, bucket, flags, (uint32_t)i, v.x, v.y, v.z);
 }
 
-// Per-cell scan + deterministic order + gather (replaces ccp's
-// calc_indexes/SortPairs/set_cells_empty/find_cells_limits sequence).
-// The block's bucket entries are staged in LDS; each entry's rank within
-// its cell (number of smaller bead indices) is then found in parallel.
+// Per-cell scan + deterministic order + gather of one or two grids
+// (replaces ccp's calc_indexes/SortPairs/set_cells_empty/find_cells_limits
+// sequence, sugrid.cu:125-133). The block's bucket entries are staged in
+// LDS; each entry's rank within its cell (number of smaller bead indices)
+// is then found in parallel.
 constexpr int ORDER_LDS = 4096;
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_order(KParams p, uint32_t *count, const uint32_t *bs_read,
-                                                      uint32_t *bs_clear, const uint32_t *bucket, const float4 *pos,
-                                                      float4 *spos, uint32_t *soi, uint32_t *cstart,
-                                                      uint32_t *flags, uint64_t *step_inc) {
+__global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, uint32_t *flags,
+                                                      uint64_t *step_inc) {
   __shared__ uint32_t red[SCAN_BLOCK / 64], wsum[SCAN_BLOCK / 64], msum[SCAN_BLOCK / 64];
   __shared__ uint32_t s_id[ORDER_LDS];
   __shared__ uint8_t s_cell[ORDER_LDS];
   __shared__ uint32_t s_loff[SCAN_BLOCK + 1];
   __shared__ uint32_t s_start[SCAN_BLOCK];
-  const int b = blockIdx.x, tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
+  const int bx = xcd_block();
+  const bool second = bx >= ja.g.nblk;
+  const OrderJob &J = second ? jb : ja;
+  const int b = second ? bx - ja.g.nblk : bx;
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
   const int c = b * SCAN_BLOCK + tid;
+  const int cap = J.g.cap, n_c = J.g.n_c;
   uint32_t part = 0;
-  for (int k = tid; k < b; k += SCAN_BLOCK) part += bs_read[k];
+  for (int k = tid; k < b; k += SCAN_BLOCK) part += J.bs_read[k];
   part = wave_sum(part);
   if (lane == 0) red[wid] = part;
-  const uint32_t cnt = (c < p.n_c) ? count[c] : 0u;
+  const uint32_t cnt = (c < n_c) ? J.count[c] : 0u;
   uint32_t m = cnt;
-  if (m > (uint32_t)p.cap) {
+  if (m > (uint32_t)cap) {
     atomicAdd(flags + FLAG_OVERFLOW, 1u);
-    m = (uint32_t)p.cap;
+    m = (uint32_t)cap;
   }
   const uint32_t incl = wave_incl_scan(cnt);
   const uint32_t mincl = wave_incl_scan(m);
@@ -456,16 +591,16 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(KParams p, uint32_t *count
   }
   const uint32_t start = prefix + woff + incl - cnt;
   moff += mincl - m;
-  if (c < p.n_c) {
-    cstart[c] = start;
-    count[c] = 0u;
+  if (c < n_c) {
+    J.cstart[c] = start;
+    J.count[c] = 0u;
   }
-  if (c == p.n_c - 1) cstart[p.n_c] = start + cnt;
+  if (c == n_c - 1) J.cstart[n_c] = start + cnt;
   if (tid == 0) {
-    bs_clear[b] = 0u;
-    if (b == 0 && step_inc) *step_inc += 1ull;
+    J.bs_clear[b] = 0u;
+    if (b == 0 && !second && step_inc) *step_inc += 1ull;
   }
-  const uint32_t *bk = bucket + (size_t)c * p.cap;
+  const uint32_t *bk = J.bucket + (size_t)c * cap;
   if (mtotal <= (uint32_t)ORDER_LDS) {
     s_loff[tid] = moff;
     s_start[tid] = start;
@@ -481,8 +616,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(KParams p, uint32_t *count
       uint32_t rank = 0;
       for (uint32_t s = b0; s < b1; ++s) rank += (s_id[s] < id) ? 1u : 0u;
       const uint32_t dst = s_start[l] + rank;
-      spos[dst] = pos[id];
-      if (soi) soi[dst] = id;
+      J.spos[dst] = J.pos[id];
+      if (J.soi) J.soi[dst] = id;
     }
   } else {
     // dense block (raw initial walks): rank each cell's entries in place
@@ -490,32 +625,34 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(KParams p, uint32_t *count
       const uint32_t e = bk[k];
       uint32_t rank = 0;
       for (uint32_t s = 0; s < m; ++s) rank += (bk[s] < e) ? 1u : 0u;
-      spos[start + rank] = pos[e];
-      if (soi) soi[start + rank] = e;
+      J.spos[start + rank] = J.pos[e];
+      if (J.soi) J.soi[start + rank] = e;
     }
   }
 }
 
-// Contact counts: pairs i < j with d < rc, FP32 without fused operations so
-// the decision matches the oracle bit for bit.
+// Contact counts on the fine grid: pairs i < j with d < rc within
+// rad = ceil(rc / csl) cells, FP32 without fused operations so the decision
+// matches the oracle bit for bit.
 __global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4 *spos, const uint32_t *cstart,
-                                                         uint32_t *map) {
+                                                         uint32_t *map, int rad) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  const int t = xcd_block() * BEAD_BLOCK + threadIdx.x;
   if (t >= p.N) return;
+  const GridGeom &g = p.gf;
   const float4 me = spos[t];
   const uint32_t i = wbits(me) & ID_MASK;
   bool outside = false;
-  const int cx = cell_coord(me.x, p, outside), cy = cell_coord(me.y, p, outside), cz = cell_coord(me.z, p, outside);
+  const int cx = cell_coord(me.x, g, outside), cy = cell_coord(me.y, g, outside), cz = cell_coord(me.z, g, outside);
   const uint32_t nb = (uint32_t)p.contact_bins, bi = i / (uint32_t)p.contact_bin_size;
-  for (int dz = -1; dz <= 1; ++dz) {
+  for (int dz = -rad; dz <= rad; ++dz) {
     const int z = cz + dz;
-    if (z < 0 || z >= p.cps) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
+    if (z < 0 || z >= g.cps) continue;
+    for (int dy = -rad; dy <= rad; ++dy) {
       const int y = cy + dy;
-      if (y < 0 || y >= p.cps) continue;
-      const int row = (z * p.cps + y) * p.cps;
-      const uint32_t lo = cstart[row + max(cx - 1, 0)], hi = cstart[row + min(cx + 1, p.cps - 1) + 1];
+      if (y < 0 || y >= g.cps) continue;
+      const int row = (z * g.cps + y) * g.cps;
+      const uint32_t lo = cstart[row + max(cx - rad, 0)], hi = cstart[row + min(cx + rad, g.cps - 1) + 1];
       for (uint32_t k = lo; k < hi; ++k) {
         const float4 o = spos[k];
         const uint32_t j = wbits(o) & ID_MASK;
@@ -535,8 +672,8 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4
   }
 }
 
-// Non-bonded pairs (|i - j| > 1) closer than dmin (particle_overlaps,
-// chrsim.cu:805-820), each pair counted once.
+// Non-bonded pairs (|i - j| > 1) closer than dmin <= fine cell side
+// (particle_overlaps, chrsim.cu:805-820), each pair counted once.
 __global__ __launch_bounds__(BEAD_BLOCK) void k_overlaps(KParams p, const float4 *spos, const uint32_t *cstart,
                                                          float dmin2, unsigned long long *count) {
   const int t = blockIdx.x * BEAD_BLOCK + threadIdx.x;
@@ -544,24 +681,15 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_overlaps(KParams p, const float4
   if (t < p.N) {
     const float4 me = spos[t];
     const uint32_t i = wbits(me) & ID_MASK;
-    bool outside = false;
-    const int cx = cell_coord(me.x, p, outside), cy = cell_coord(me.y, p, outside), cz = cell_coord(me.z, p, outside);
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int z = cz + dz;
-      if (z < 0 || z >= p.cps) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int y = cy + dy;
-        if (y < 0 || y >= p.cps) continue;
-        const int row = (z * p.cps + y) * p.cps;
-        const uint32_t lo = cstart[row + max(cx - 1, 0)], hi = cstart[row + min(cx + 1, p.cps - 1) + 1];
-        for (uint32_t k = lo; k < hi; ++k) {
-          const float4 o = spos[k];
-          const uint32_t j = wbits(o) & ID_MASK;
-          const float dx = me.x - o.x, dy2 = me.y - o.y, dz2 = me.z - o.z;
-          if (j > i + 1u && dx * dx + dy2 * dy2 + dz2 * dz2 < dmin2) ++n;
-        }
+    Rows R;
+    neighbour_rows(p.gf, cstart, me.x, me.y, me.z, R);
+    for (int r = 0; r < 9; ++r)
+      for (uint32_t k = R.lo[r]; k < R.hi[r]; ++k) {
+        const float4 o = spos[k];
+        const uint32_t j = wbits(o) & ID_MASK;
+        const float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+        if (j > i + 1u && dx * dx + dy * dy + dz * dz < dmin2) ++n;
       }
-    }
   }
   const uint32_t tot = wave_sum(n);
   if (__lane_id() == 0 && tot) atomicAdd(count, (unsigned long long)tot);
@@ -579,42 +707,43 @@ inline unsigned blocks_for(int n, int bs) { return (unsigned)((n + bs - 1) / bs)
 
 } // namespace
 
-void launch_bin(hipStream_t s, const KParams &p, int n_obj, const float4 *pos, uint32_t *count, uint32_t *blocksum,
-                uint32_t *bucket, uint32_t *flags) {
+void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, const float4 *pos, bool het_only,
+                uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags) {
+  (void)p;
   if (n_obj <= 0) return;
-  KPtrs q{};
-  q.count = count;
-  q.blocksum = blocksum;
-  q.bucket = bucket;
-  q.flags = flags;
-  k_bin<<<blocks_for(n_obj, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, n_obj, pos, q);
+  k_bin<<<blocks_for(n_obj, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(g, n_obj, pos, het_only ? 1 : 0, count, blocksum,
+                                                            bucket, flags);
 }
 
-void launch_order(hipStream_t s, const KParams &p, int n_obj, uint32_t *count, const uint32_t *blocksum_read,
-                  uint32_t *blocksum_clear, const uint32_t *bucket, const float4 *pos, float4 *spos, uint32_t *soi,
-                  uint32_t *cstart, uint32_t *flags, uint64_t *step_inc) {
-  (void)n_obj;
-  k_order<<<(unsigned)p.nblk, SCAN_BLOCK, 0, s>>>(p, count, blocksum_read, blocksum_clear, bucket, pos, spos, soi,
-                                                  cstart, flags, step_inc);
+void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc) {
+  const OrderJob jb = b ? *b : a;
+  const unsigned blocks = (unsigned)(a.g.nblk + (b ? b->g.nblk : 0));
+  k_order<<<blocks, SCAN_BLOCK, 0, s>>>(a, jb, flags, step_inc);
 }
 
-template <int MODE> static void force_mode(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
+template <int MODE, bool SPLIT> static void force_mode(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
   const unsigned g = blocks_for(p.N, BEAD_BLOCK);
-  if (stage == 0) k_force<MODE, 0><<<g, BEAD_BLOCK, 0, s>>>(p, q);
-  else if (stage == 1) k_force<MODE, 1><<<g, BEAD_BLOCK, 0, s>>>(p, q);
-  else k_force<MODE, 2><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  if (stage == 0) k_force<MODE, 0, SPLIT><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else if (stage == 1) k_force<MODE, 1, SPLIT><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else k_force<MODE, 2, SPLIT><<<g, BEAD_BLOCK, 0, s>>>(p, q);
 }
 
 void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q) {
   if (p.N <= 0) return;
-  if (mode == MODE_CCP) force_mode<MODE_CCP>(s, stage, p, q);
-  else if (mode == MODE_ICG) force_mode<MODE_ICG>(s, stage, p, q);
-  else force_mode<MODE_LJ>(s, stage, p, q);
+  if (mode == MODE_ICG) force_mode<MODE_ICG, false>(s, stage, p, q);
+  else if (p.split) {
+    if (mode == MODE_CCP) force_mode<MODE_CCP, true>(s, stage, p, q);
+    else force_mode<MODE_LJ, true>(s, stage, p, q);
+  } else {
+    if (mode == MODE_CCP) force_mode<MODE_CCP, false>(s, stage, p, q);
+    else force_mode<MODE_LJ, false>(s, stage, p, q);
+  }
 }
 
 void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map) {
   if (p.N <= 0) return;
-  k_contacts<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, map);
+  const int rad = (int)ceilf(sqrtf(p.contact_rc2) / p.gf.csl);
+  k_contacts<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, map, rad);
 }
 
 void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, float dmin2,

@@ -28,10 +28,30 @@ enum : int { FLAG_OVERFLOW = 0, FLAG_OUTSIDE = 1, FLAG_NONFINITE = 2, FLAG_COUNT
 
 enum Mode : int { MODE_CCP = 0, MODE_ICG = 1, MODE_LJ = 2 };
 
+// Geometry of one uniform cell grid centred on the nucleus (ccp's sugrid,
+// sugrid.cuh:15-54): cell (gx, gy, gz) = floor(r / csl) + cps/2, index
+// gx + cps (gy + cps gz).
+struct GridGeom {
+  float csl;
+  int cps, half, n_c, cap, nblk;
+};
+
+// Device buffers of one grid. count/blocksum/bucket receive the binning of
+// the next positions; cstart/spos/soi hold the ordered cell list.
+struct GridBufs {
+  uint32_t *count;
+  uint32_t *bs[2];   // per-scan-block totals, double buffered
+  uint32_t *bucket;  // n_c * cap unsorted members
+  uint32_t *cstart;  // n_c + 1 starts into spos
+  float4 *spos;      // positions in cell order
+  uint32_t *soi;     // bead index in cell order (optional)
+};
+
 struct KParams {
   int N, n_l;
-  int cps, half, n_c, cap, nblk;
-  float csl;
+  GridGeom gf; // grid of every bead (all pairs, or the short-range pass when split), LBS sites
+  GridGeom gh; // grid of het beads (long-range het-het pass when split)
+  int split;   // 1: short-range pairs on gf + het-het tail on gh; 0: every pair on gf
   // nucleus geometry (cngeom, chrdat.cuh:48-69)
   float R_n, R_o, R_b, noc, boc, nod, bod;
   // bonds and bending
@@ -54,26 +74,38 @@ struct KParams {
 };
 
 struct KPtrs {
-  const float4 *spos;     // bead positions sorted by cell
-  const uint32_t *cstart; // n_c + 1 cell starts into spos
-  const float4 *lspos;    // LBS positions sorted by cell
-  const uint32_t *lstart; // n_c + 1
-  const float4 *pos;      // bead positions of the evaluated stage, by bead index
-  float4 *out0;           // stage 1: er, stage 2: new r, stage 0: force
-  float4 *pstar;          // r + f dt/2 + noise (written by stage 1, read by stage 2)
-  uint32_t *count;        // n_c per-cell counters (zero between builds)
-  uint32_t *blocksum;     // nblk per-scan-block counters being accumulated
-  uint32_t *bucket;       // n_c * cap unsorted cell members
-  uint32_t *flags;        // FLAG_COUNT error counters
-  const uint64_t *step;   // global step counter (noise counter)
+  const float4 *fspos;     // fine grid of the evaluated positions
+  const uint32_t *fcstart;
+  const float4 *hspos;     // het grid of the evaluated positions
+  const uint32_t *hcstart;
+  const float4 *lspos;     // LBS sites on the fine geometry
+  const uint32_t *lstart;
+  const float4 *pos;       // bead positions of the evaluated stage, by bead index
+  float4 *out0;            // stage 1: er, stage 2: new r, stage 0: force
+  float4 *pstar;           // r + f dt/2 + noise (written by stage 1, read by stage 2)
+  uint32_t *fcount, *fblocksum, *fbucket; // binning of the output into the next fine grid
+  uint32_t *hcount, *hblocksum, *hbucket; // ... and into the next het grid
+  uint32_t *flags;         // FLAG_COUNT error counters
+  const uint64_t *step;    // global step counter (noise counter)
+};
+
+// One grid's ordering work inside a k_order launch.
+struct OrderJob {
+  GridGeom g;
+  uint32_t *count;
+  const uint32_t *bs_read;
+  uint32_t *bs_clear;
+  const uint32_t *bucket;
+  const float4 *pos;
+  float4 *spos;
+  uint32_t *soi;
+  uint32_t *cstart;
 };
 
 // Launchers (kernels.hip). All are asynchronous on `s`.
-void launch_bin(hipStream_t s, const KParams &p, int n_obj, const float4 *pos, uint32_t *count, uint32_t *blocksum,
-                uint32_t *bucket, uint32_t *flags);
-void launch_order(hipStream_t s, const KParams &p, int n_obj, uint32_t *count, const uint32_t *blocksum_read,
-                  uint32_t *blocksum_clear, const uint32_t *bucket, const float4 *pos, float4 *spos, uint32_t *soi,
-                  uint32_t *cstart, uint32_t *flags, uint64_t *step_inc);
+void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, const float4 *pos, bool het_only,
+                uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags);
+void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc);
 void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q);
 void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map);
 void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, float dmin2,
