@@ -170,6 +170,24 @@ int ccp_contact_sample(ccp_sim *h);
 int ccp_contact_reset(ccp_sim *h);
 int ccp_contact_map(ccp_sim *h, uint32_t *map, size_t n_bins2, uint32_t *n_samples);
 
+/* ccp's per-frame observables of the current positions, computed on the
+ * device (chrana::calc_observables, chrana.cu:372-480): scalars[0] centre of
+ * mass distance, [1] radius of gyration squared, [2] nematic order
+ * parameter, [3] nucleus chromatin fraction; rcd (3 x n_bins, row-major)
+ * the radial chromatin density of A beads, B beads and all beads (ccp uses
+ * 64 bins). */
+int ccp_observables(ccp_sim *h, double scalars[4], double *rcd, size_t n_bins);
+
+/* Mean distance d(s) and contact probability P(s) (d < 2) over the bead
+ * pairs (p, p+s) of chain `chain`, s = 1..s_max (calc_sd_cp,
+ * chrana.cu:13-45, for one chromosome copy, without ccp's factor 1000). */
+int ccp_distance_contact(ccp_sim *h, uint32_t chain, uint32_t s_max, double *sd, double *cp);
+
+/* ccp's per-frame contact map of the current positions (calc_cm,
+ * chrana.cu:48-84): cms (cms+1)/2 floats of the packed lower triangle, with
+ * cms = N/4 (N/8 when N = 30386, averaging the two chromosome copies). */
+int ccp_frame_contact_map(ccp_sim *h, float *cm, size_t lcm);
+
 /* Average device time per launch (ms) of each kernel of the step over
  * n_steps steps, measured with HIP events on the handle's stream:
  * out[0] force stage 1, [1] order (scan + sort + gather) after stage 1,

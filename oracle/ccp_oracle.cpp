@@ -622,6 +622,73 @@ int orc_contacts_grid(const float *r, int32_t n, float rc, int32_t bin_size, int
   return 0;
 }
 
+int orc_observables(const double *r, const uint8_t *flags, int32_t n, double R_n, double nod, int32_t n_b,
+                    double out[4], double *rcd) {
+  auto R3 = [&](int i) { return V3<double>{r[3 * i], r[3 * i + 1], r[3 * i + 2]}; };
+  V3<double> cm = {0, 0, 0};
+  for (int i = 0; i < n; ++i) cm += R3(i);
+  cm = cm * (1.0 / n);
+  double rg2 = 0;
+  for (int i = 0; i < n; ++i) rg2 += dot(R3(i) - cm, R3(i) - cm);
+  double nop = 0;
+  for (int i = 0; i + 1 < n; ++i) {
+    V3<double> v = R3(i + 1) - R3(i);
+    double c = dot(v, R3(i)) / (len(v) * len(R3(i)));
+    nop += 0.5 * (3.0 * c * c - 1.0);
+  }
+  double ncf = 0;
+  for (int i = 0; i < n; ++i) ncf += (r[3 * i + 2] < nod) ? 1.0 : 0.0;
+  out[0] = len(cm);
+  out[1] = rg2 / n;
+  out[2] = n > 1 ? nop / (n - 1.0) : 0.0;
+  out[3] = ncf / n;
+  const double a = (double)1.154701f / 2.0, vb = 4.0 / 3.0 * M_PI * a * a * a;
+  std::vector<double> rb(n_b);
+  for (int k = 0; k < n_b; ++k) rb[k] = R_n * std::pow((k + 1.0) / n_b, 1.0 / 3.0);
+  std::fill(rcd, rcd + 3 * n_b, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double d = len(R3(i));
+    int t = (flags[i] & ORC_HET) ? 0 : 1;
+    for (int k = 0; k < n_b; ++k) {
+      double v;
+      if (d - a > rb[k]) v = 0;
+      else if (d + a < rb[k]) v = vb;
+      else {
+        double ca = (d * d + a * a - rb[k] * rb[k]) / (2.0 * d * a);
+        double cb = (rb[k] * rb[k] + d * d - a * a) / (2.0 * rb[k] * d);
+        v = M_PI / 3.0 * a * a * a * (2.0 + ca) * (1.0 - ca) * (1.0 - ca) +
+            M_PI / 3.0 * rb[k] * rb[k] * rb[k] * (2.0 + cb) * (1.0 - cb) * (1.0 - cb);
+      }
+      rcd[t * n_b + k] += v;
+      rcd[2 * n_b + k] += v;
+      if (k != n_b - 1) {
+        rcd[t * n_b + k + 1] -= v;
+        rcd[2 * n_b + k + 1] -= v;
+      }
+    }
+  }
+  const double shell = 4.0 / 3.0 * M_PI * R_n * R_n * R_n / n_b;
+  for (int k = 0; k < 3 * n_b; ++k) rcd[k] /= shell;
+  return 0;
+}
+
+int orc_sd_cp(const double *r, int32_t start, int32_t end, int32_t s_max, double *sd, double *cp) {
+  for (int s = 1; s <= s_max; ++s) {
+    double sum = 0, cnt = 0;
+    const int m = end - start - s;
+    for (int p = start; p < end - s; ++p) {
+      double dx = r[3 * (p + s)] - r[3 * p], dy = r[3 * (p + s) + 1] - r[3 * p + 1],
+             dz = r[3 * (p + s) + 2] - r[3 * p + 2];
+      double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+      sum += d;
+      if (d < 2.0) cnt += 1.0;
+    }
+    sd[s - 1] = m > 0 ? sum / m : 0.0;
+    cp[s - 1] = m > 0 ? cnt / m : 0.0;
+  }
+  return 0;
+}
+
 int orc_ccp_contact_map(const float *r, int32_t n, float *cm) {
   const uint32_t N = (uint32_t)n, N_def = 30386, px = 4;
   const float aco = 2.0f, cf = 1000.0f;

@@ -118,6 +118,19 @@ int orc_contacts_grid(const float *r, int32_t n, float rc, int32_t bin_size,
  * two-copy average when n == 30386. */
 int orc_ccp_contact_map(const float *r, int32_t n, float *cm);
 
+/* ccp's per-frame observables (chrana.cu:372-480) in FP64 for n beads:
+ * out[0] centre-of-mass distance, [1] radius of gyration squared,
+ * [2] nematic order parameter, [3] fraction with z < nod; rcd (3 x n_b) the
+ * radial chromatin density of A beads (HET flag), B beads and all beads
+ * with the lens-volume formula of ccp. */
+int orc_observables(const double *r, const uint8_t *flags, int32_t n, double R_n, double nod, int32_t n_b,
+                    double out[4], double *rcd);
+
+/* Mean distance d(s) and contact probability P(s) (d < 2) over the pairs
+ * (p, p+s) of beads [start, end), s = 1..s_max (calc_sd_cp,
+ * chrana.cu:13-45, one copy, without ccp's factor 1000). */
+int orc_sd_cp(const double *r, int32_t start, int32_t end, int32_t s_max, double *sd, double *cp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,9 @@ def lib():
         L.orc_contacts.argtypes = [f32p, ctypes.c_int32, ctypes.c_float, ctypes.c_int32, ctypes.c_int32, u32p]
         L.orc_noise_batch.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int32, f64p, P(ctypes.c_int32)]
         L.orc_contacts_grid.argtypes = [f32p, ctypes.c_int32, ctypes.c_float, ctypes.c_int32, ctypes.c_int32, u32p]
+        L.orc_observables.argtypes = [f64p, P(ctypes.c_uint8), ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int32, f64p, f64p]
+        L.orc_sd_cp.argtypes = [f64p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, f64p, f64p]
         L.orc_ccp_contact_map.argtypes = [f32p, ctypes.c_int32, f32p]
         _lib = L
     return _lib
@@ -184,3 +187,21 @@ def ccp_contact_map(r):
     cm = np.zeros(cms * (cms + 1) // 2, np.float32)
     lib().orc_ccp_contact_map(_p(r, ctypes.c_float), n, _p(cm, ctypes.c_float))
     return cm
+
+
+def observables(r, flags, R_n, nod=None, n_b=64):
+    """ccp's frame observables: (dcm, rg2, nop, ncf), rcd[3, n_b]."""
+    r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    out = np.zeros(4)
+    rcd = np.zeros((3, n_b))
+    lib().orc_observables(_p(r, ctypes.c_double), _p(fl, ctypes.c_uint8), r.shape[0], R_n,
+                          R_n if nod is None else nod, n_b, _p(out, ctypes.c_double), _p(rcd, ctypes.c_double))
+    return out, rcd
+
+
+def sd_cp(r, start, end, s_max):
+    r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)
+    sd, cp = np.zeros(s_max), np.zeros(s_max)
+    lib().orc_sd_cp(_p(r, ctypes.c_double), start, end, s_max, _p(sd, ctypes.c_double), _p(cp, ctypes.c_double))
+    return sd, cp

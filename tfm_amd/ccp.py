@@ -26,7 +26,8 @@ EXPORTS = (
     "ccp_synchronize", "ccp_get_time", "ccp_set_time", "ccp_get_positions", "ccp_set_positions",
     "ccp_get_types", "ccp_set_types", "ccp_get_flags", "ccp_set_flags", "ccp_get_lbs", "ccp_set_lbs",
     "ccp_get_chains", "ccp_compute_forces", "ccp_cell_list", "ccp_noise", "ccp_count_overlaps",
-    "ccp_contact_sample", "ccp_contact_reset", "ccp_contact_map", "ccp_time_kernels",
+    "ccp_contact_sample", "ccp_contact_reset", "ccp_contact_map", "ccp_time_kernels", "ccp_observables",
+    "ccp_distance_contact", "ccp_frame_contact_map",
 )
 
 
@@ -92,6 +93,10 @@ def lib():
         L.ccp_count_overlaps.argtypes = [vp, ctypes.c_float, P(ctypes.c_uint64)]
         L.ccp_contact_map.argtypes = [vp, u32p, sz, u32p]
         L.ccp_time_kernels.argtypes = [vp, ctypes.c_uint32, f32p]
+        f64p = P(ctypes.c_double)
+        L.ccp_observables.argtypes = [vp, f64p, f64p, sz]
+        L.ccp_distance_contact.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, f64p, f64p]
+        L.ccp_frame_contact_map.argtypes = [vp, f32p, sz]
         _lib = L
     return _lib
 
@@ -286,3 +291,22 @@ class Simulation:
         out = np.zeros(5, np.float32)
         self._check(lib().ccp_time_kernels(self.h, n_steps, _ptr(out, ctypes.c_float)))
         return out
+
+    def observables(self, n_bins=64):
+        """ccp's frame observables: ((dcm, rg2, nop, ncf), rcd[3, n_bins])."""
+        sc = np.zeros(4)
+        rcd = np.zeros((3, n_bins))
+        self._check(lib().ccp_observables(self.h, _ptr(sc, ctypes.c_double), _ptr(rcd, ctypes.c_double), n_bins))
+        return sc, rcd
+
+    def distance_contact(self, chain, s_max):
+        sd, cp = np.zeros(s_max), np.zeros(s_max)
+        self._check(lib().ccp_distance_contact(self.h, chain, s_max, _ptr(sd, ctypes.c_double),
+                                               _ptr(cp, ctypes.c_double)))
+        return sd, cp
+
+    def frame_contact_map(self):
+        cms = (self.N // 2) // 4 if self.N == 30386 else self.N // 4
+        cm = np.zeros(cms * (cms + 1) // 2, np.float32)
+        self._check(lib().ccp_frame_contact_map(self.h, _ptr(cm, ctypes.c_float), cm.size))
+        return cm

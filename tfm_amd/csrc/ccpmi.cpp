@@ -1466,6 +1466,60 @@ int ccp_contact_map(ccp_sim *h, uint32_t *map, size_t n_bins2, uint32_t *n_sampl
   });
 }
 
+int ccp_observables(ccp_sim *h, double scalars[4], double *rcd, size_t n_bins) {
+  return guarded(h, [&] {
+    need(scalars && rcd && n_bins >= 1 && n_bins <= 4096, "observables size mismatch");
+    const size_t n_acc = 6 + 3 * n_bins;
+    DevBuf<double> acc;
+    acc.alloc(n_acc);
+    hip_check(hipMemsetAsync(acc.p, 0, n_acc * sizeof(double), h->stream));
+    launch_observables(h->stream, (int)h->N, (double)h->R_n, (double)h->nod, (double)RCO / 2.0, (int)n_bins, h->r.p,
+                       acc.p);
+    hip_check(hipGetLastError());
+    std::vector<double> a(n_acc);
+    hip_check(hipMemcpyAsync(a.data(), acc.p, n_acc * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    acc.free_();
+    const double n = h->N, cx = a[0] / n, cy = a[1] / n, cz = a[2] / n;
+    scalars[0] = std::sqrt(cx * cx + cy * cy + cz * cz);
+    scalars[1] = a[3] / n - (cx * cx + cy * cy + cz * cz);
+    scalars[2] = h->N > 1 ? a[4] / (n - 1.0) : 0.0;
+    scalars[3] = a[5] / n;
+    const double shell = 4.0 / 3.0 * M_PI * (double)h->R_n * h->R_n * h->R_n / (double)n_bins;
+    for (size_t k = 0; k < 3 * n_bins; ++k) rcd[k] = a[6 + k] / shell;
+  });
+}
+
+int ccp_distance_contact(ccp_sim *h, uint32_t chain, uint32_t s_max, double *sd, double *cp) {
+  return guarded(h, [&] {
+    need(sd && cp && chain + 1 < h->chain_off.size(), "bad chain");
+    const uint32_t b = h->chain_off[chain], e = h->chain_off[chain + 1];
+    need(s_max >= 1 && s_max < e - b, "s_max out of range");
+    DevBuf<double> out;
+    out.alloc(2 * (size_t)s_max);
+    launch_sd_cp(h->stream, h->r.p, (int)b, (int)e, (int)s_max, out.p, out.p + s_max);
+    hip_check(hipGetLastError());
+    hip_check(hipMemcpyAsync(sd, out.p, s_max * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipMemcpyAsync(cp, out.p + s_max, s_max * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    out.free_();
+  });
+}
+
+int ccp_frame_contact_map(ccp_sim *h, float *cm, size_t lcm) {
+  return guarded(h, [&] {
+    const uint32_t cms = (h->N == N_DEF) ? (h->N / 2) / 4 : h->N / 4;
+    need(cm && lcm == (size_t)cms * (cms + 1) / 2, "contact map size mismatch");
+    DevBuf<float> out;
+    out.alloc(lcm);
+    launch_ccp_cm(h->stream, h->N, (uint32_t)lcm, h->r.p, out.p);
+    hip_check(hipGetLastError());
+    hip_check(hipMemcpyAsync(cm, out.p, lcm * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    out.free_();
+  });
+}
+
 int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]) {
   return guarded(h, [&] {
     need(out && n_steps > 0, "bad timing request");

@@ -427,14 +427,38 @@ __device__ __forceinline__ int xcd_block() {
   return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + k;
 }
 
-// Candidates of the 9 rows, four at a time (independent loads in flight).
+// The 3 neighbour rows (y-1..y+1) of plane z + dz around the cell holding
+// (x, y, z); rows outside the grid get an empty range.
+struct Rows3 {
+  uint32_t lo[3], hi[3];
+  int c0[3], c1[3];
+};
+
+__device__ __forceinline__ void plane_rows(const GridGeom &g, const uint32_t *__restrict__ cstart, float x, float y,
+                                           float z, int dz, Rows3 &R) {
+  bool outside = false;
+  const int cx = cell_coord(x, g, outside), cy = cell_coord(y, g, outside), cz = cell_coord(z, g, outside);
+  const int zz = cz + dz;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int yy = cy + r - 1;
+    const bool ok = zz >= 0 && zz < g.cps && yy >= 0 && yy < g.cps;
+    const int row = (zz * g.cps + yy) * g.cps;
+    R.c0[r] = ok ? row + max(cx - 1, 0) : 0;
+    R.c1[r] = ok ? row + min(cx + 1, g.cps - 1) + 1 : 0;
+    R.lo[r] = cstart[R.c0[r]];
+    R.hi[r] = cstart[R.c1[r]];
+  }
+}
+
+// Candidates of 3 rows, four at a time (independent loads in flight).
 // PASS 0: every pair on the all-bead grid (full cutoffs, or the short
 // cutoff when split); PASS 1: het-het tail on the het grid.
 template <int MODE, int PASS, bool SPLIT>
-__device__ __forceinline__ void scan_pass(const KParams &p, const float4 *__restrict__ spos, const Rows &R, float rx,
+__device__ __forceinline__ void scan_pass(const KParams &p, const float4 *__restrict__ spos, const Rows3 &R, float rx,
                                           float ry, float rz, uint32_t w, float &fx, float &fy, float &fz) {
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
+  for (int r = 0; r < 3; ++r) {
     uint32_t k = R.lo[r];
     const uint32_t h = R.hi[r];
     for (; k + 4u <= h; k += 4u) {
@@ -464,42 +488,62 @@ __device__ __forceinline__ void scan_pass(const KParams &p, const float4 *__rest
   }
 }
 
-// Force evaluation of one stage: one thread per bead in grid order,
-// candidates from the 9 contiguous neighbour rows of the all-bead grid
-// (and, when split, het-het tails from the het grid).
+// Three lanes per bead: lane part p scans the neighbour rows of plane
+// z + p - 1; the partial forces are summed with two shuffles. 21 beads per
+// wavefront (lane 63 idles), so N beads give 3N/63 wavefronts to hide the
+// gather latency.
+constexpr int BEADS_PER_WAVE = 21;
+constexpr int BEADS_PER_BLOCK = BEADS_PER_WAVE * (BEAD_BLOCK / 64);
+
+// Force evaluation of one stage, beads in grid order.
 // STAGE 0: write F(pos) by bead index. STAGE 1: Heun predictor
 // (exec_RK_1, chrsim.cu:364-388) er = r + f dt + rn, keeps
 // r + f dt/2 + rn. STAGE 2: corrector (exec_RK_2, :391-416)
 // r = (r + f dt/2 + rn) + ef dt/2. Stages 1 and 2 bin their output.
 template <int MODE, int STAGE, bool SPLIT>
 __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
-  const int t = xcd_block() * BEAD_BLOCK + threadIdx.x;
-  if (t >= p.N) return;
-  const float4 me = q.fspos[t];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const int part = lane % 3, slot = lane / 3;
+  const int t = (xcd_block() * (BEAD_BLOCK / 64) + wave) * BEADS_PER_WAVE + slot;
+  const bool active = slot < BEADS_PER_WAVE && t < p.N;
+  const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const uint32_t w = wbits(me);
   const uint32_t i = w & ID_MASK;
   const float rx = me.x, ry = me.y, rz = me.z;
 
-  Rows R;
-  neighbour_rows(p.gf, q.fcstart, rx, ry, rz, R);
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  bonded_force(p, q.pos, i, w, rx, ry, rz, fx, fy, fz);
-  wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
-  scan_pass<MODE, 0, SPLIT>(p, q.fspos, R, rx, ry, rz, w, fx, fy, fz);
-  if (MODE != MODE_ICG && p.n_l > 0 && (w & W_LAD)) {
+  Rows3 R;
+  plane_rows(p.gf, q.fcstart, rx, ry, rz, part - 1, R);
+  if (!active)
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
+    for (int r = 0; r < 3; ++r) R.hi[r] = R.lo[r];
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (active && part == 0) {
+    bonded_force(p, q.pos, i, w, rx, ry, rz, fx, fy, fz);
+    wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
+  }
+  scan_pass<MODE, 0, SPLIT>(p, q.fspos, R, rx, ry, rz, w, fx, fy, fz);
+  if (MODE != MODE_ICG && active && p.n_l > 0 && (w & W_LAD)) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
       const uint32_t llo = q.lstart[R.c0[r]], lhi = q.lstart[R.c1[r]];
       for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
     }
   }
   if constexpr (SPLIT && MODE != MODE_ICG) {
-    if (w & W_HET) {
-      Rows H;
-      neighbour_rows(p.gh, q.hcstart, rx, ry, rz, H);
+    if (active && (w & W_HET)) {
+      Rows3 H;
+      plane_rows(p.gh, q.hcstart, rx, ry, rz, part - 1, H);
       scan_pass<MODE, 1, true>(p, q.hspos, H, rx, ry, rz, w, fx, fy, fz);
     }
   }
+  // sum the three parts into lane part 0
+  const int s1 = min(lane + 1, 63), s2 = min(lane + 2, 63);
+  const float ax = __shfl(fx, s1), ay = __shfl(fy, s1), az = __shfl(fz, s1);
+  const float bx = __shfl(fx, s2), by = __shfl(fy, s2), bz = __shfl(fz, s2);
+  if (!active || part != 0) return;
+  fx += ax + bx;
+  fy += ay + by;
+  fz += az + bz;
   if (STAGE != 0 && p.fcap2 > 0.0f) {
     // optional safeguard of the WCA/LJ set: |F| <= fcap
     const float m2 = fx * fx + fy * fy + fz * fz;
@@ -546,7 +590,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_bin(GridGeom g, int n_obj, const
 // sequence, sugrid.cu:125-133). The block's bucket entries are staged in
 // LDS; each entry's rank within its cell (number of smaller bead indices)
 // is then found in parallel.
-constexpr int ORDER_LDS = 4096;
+constexpr int ORDER_LDS = 2048;
 
 __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, uint32_t *flags,
                                                       uint64_t *step_inc) {
@@ -703,6 +747,107 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_noise(KParams p, uint64_t step, 
   out[i] = make_float4(z0, z1, z2, __uint_as_float(a));
 }
 
+// ccp's per-frame observables (chrana.cu:372-480) summed in FP64: out[0..2]
+// sum of r, [3] sum of |r|^2, [4] sum of the bond order P2(cos), [5] beads
+// with z < nod, then the radial density of A, B and all beads (3 x n_b,
+// lens volume of a bead of radius a inside the sphere of each bin edge,
+// differenced bin by bin).
+__global__ __launch_bounds__(BEAD_BLOCK) void k_observables(int N, double R_n, double nod, double a, int n_b,
+                                                            const float4 *pos, double *out) {
+  extern __shared__ double s_acc[];
+  const int tid = threadIdx.x, n_acc = 6 + 3 * n_b;
+  for (int k = tid; k < n_acc; k += BEAD_BLOCK) s_acc[k] = 0.0;
+  __syncthreads();
+  const int i = blockIdx.x * BEAD_BLOCK + tid;
+  if (i < N) {
+    const float4 q = pos[i];
+    const double x = q.x, y = q.y, z = q.z;
+    atomicAdd(&s_acc[0], x);
+    atomicAdd(&s_acc[1], y);
+    atomicAdd(&s_acc[2], z);
+    const double d2 = x * x + y * y + z * z;
+    atomicAdd(&s_acc[3], d2);
+    if (i + 1 < N) {
+      const float4 n = pos[i + 1];
+      const double vx = (double)n.x - x, vy = (double)n.y - y, vz = (double)n.z - z;
+      const double c = (vx * x + vy * y + vz * z) / (sqrt(vx * vx + vy * vy + vz * vz) * sqrt(d2));
+      atomicAdd(&s_acc[4], 0.5 * (3.0 * c * c - 1.0));
+    }
+    if (z < nod) atomicAdd(&s_acc[5], 1.0);
+    const double d = sqrt(d2), vb = 4.0 / 3.0 * M_PI * a * a * a;
+    const int t = (wbits(q) & W_HET) ? 0 : 1;
+    double prev = 0.0;
+    for (int k = 0; k < n_b; ++k) {
+      const double rb = R_n * cbrt((k + 1.0) / n_b);
+      double v;
+      if (d - a > rb) v = 0.0;
+      else if (d + a < rb) v = vb;
+      else {
+        const double ca = (d * d + a * a - rb * rb) / (2.0 * d * a);
+        const double cb = (rb * rb + d * d - a * a) / (2.0 * rb * d);
+        v = M_PI / 3.0 * a * a * a * (2.0 + ca) * (1.0 - ca) * (1.0 - ca) +
+            M_PI / 3.0 * rb * rb * rb * (2.0 + cb) * (1.0 - cb) * (1.0 - cb);
+      }
+      const double net = v - prev;
+      prev = v;
+      if (net != 0.0) {
+        atomicAdd(&s_acc[6 + t * n_b + k], net);
+        atomicAdd(&s_acc[6 + 2 * n_b + k], net);
+      }
+      if (v == vb) break; // the rest of the bins see no change
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < n_acc; k += BEAD_BLOCK)
+    if (s_acc[k] != 0.0) atomicAdd(&out[k], s_acc[k]);
+}
+
+// Mean distance and contact fraction (d < 2) over pairs (p, p+s) of beads
+// [start, end), one thread per separation s (calc_sd_cp, chrana.cu:13-45).
+__global__ __launch_bounds__(BEAD_BLOCK) void k_sd_cp(const float4 *pos, int start, int end, int s_max, double *sd,
+                                                      double *cp) {
+  const int s = blockIdx.x * BEAD_BLOCK + threadIdx.x + 1;
+  if (s > s_max) return;
+  double sum = 0.0, cnt = 0.0;
+  for (int p = start; p < end - s; ++p) {
+    const float4 u = pos[p], v = pos[p + s];
+    const double dx = (double)v.x - u.x, dy = (double)v.y - u.y, dz = (double)v.z - u.z;
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    sum += d;
+    if (d < 2.0) cnt += 1.0;
+  }
+  const int m = end - start - s;
+  sd[s - 1] = m > 0 ? sum / m : 0.0;
+  cp[s - 1] = m > 0 ? cnt / m : 0.0;
+}
+
+// ccp's per-frame contact map (calc_cm, chrana.cu:48-84) with its exact
+// float arithmetic: pixel i_a of the packed lower triangle, 4-bead pixels,
+// contact if |r_p - r_q| < 2, factor 1000, average over the two chromosome
+// copies when N = 30386.
+__global__ __launch_bounds__(BEAD_BLOCK) void k_ccp_cm(uint32_t N, uint32_t lcm, const float4 *pos, float *cm) {
+#pragma clang fp contract(off)
+  const uint32_t ia = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (ia >= lcm) return;
+  const uint32_t px = 4, ncc = (N == 30386u) ? 2u : 1u;
+  const uint32_t ixp = (uint32_t)((double)sqrtf((float)(2.0 * ia + 0.25)) - 0.5);
+  const uint32_t iyp = (uint32_t)(ia - ixp * (ixp + 1.0) / 2.0);
+  uint32_t sx = ixp * px, ex = sx + px, sy = iyp * px, ey = sy + px;
+  float acc = 0.0f;
+  for (uint32_t c = 0; c < ncc; ++c) {
+    for (uint32_t p = sx; p < ex; ++p)
+      for (uint32_t q = sy; q < ey; ++q) {
+        const float4 u = pos[p], v = pos[q];
+        const float dx = u.x - v.x, dy = u.y - v.y, dz = u.z - v.z;
+        const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+        const float d = sqrtf((xx + yy) + zz);
+        if (d < 2.0f) acc += 1000.0f;
+      }
+    sx += N / ncc; ex += N / ncc; sy += N / ncc; ey += N / ncc;
+  }
+  cm[ia] = acc / (float)(ncc * px * px);
+}
+
 inline unsigned blocks_for(int n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 } // namespace
@@ -722,7 +867,7 @@ void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t 
 }
 
 template <int MODE, bool SPLIT> static void force_mode(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
-  const unsigned g = blocks_for(p.N, BEAD_BLOCK);
+  const unsigned g = blocks_for(p.N, BEADS_PER_BLOCK);
   if (stage == 0) k_force<MODE, 0, SPLIT><<<g, BEAD_BLOCK, 0, s>>>(p, q);
   else if (stage == 1) k_force<MODE, 1, SPLIT><<<g, BEAD_BLOCK, 0, s>>>(p, q);
   else k_force<MODE, 2, SPLIT><<<g, BEAD_BLOCK, 0, s>>>(p, q);
@@ -755,6 +900,23 @@ void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const 
 void launch_noise(hipStream_t s, const KParams &p, uint64_t step, float4 *out) {
   if (p.N <= 0) return;
   k_noise<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, step, out);
+}
+
+void launch_observables(hipStream_t s, int N, double R_n, double nod, double a, int n_b, const float4 *pos,
+                        double *out) {
+  if (N <= 0) return;
+  k_observables<<<blocks_for(N, BEAD_BLOCK), BEAD_BLOCK, (6 + 3 * n_b) * sizeof(double), s>>>(N, R_n, nod, a, n_b,
+                                                                                               pos, out);
+}
+
+void launch_sd_cp(hipStream_t s, const float4 *pos, int start, int end, int s_max, double *sd, double *cp) {
+  if (s_max <= 0) return;
+  k_sd_cp<<<blocks_for(s_max, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(pos, start, end, s_max, sd, cp);
+}
+
+void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm) {
+  if (lcm == 0) return;
+  k_ccp_cm<<<blocks_for((int)lcm, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(N, lcm, pos, cm);
 }
 
 } // namespace ccpmi

@@ -18,8 +18,8 @@ constexpr uint32_t W_B1 = 1u << 29; // r[i]   - r[i-1]
 constexpr uint32_t W_B2 = 1u << 30; // r[i+1] - r[i]
 constexpr uint32_t W_B3 = 1u << 31; // r[i+2] - r[i+1]
 
-// cells per scan block of the order kernel
-constexpr int SCAN_BLOCK = 256;
+// cells per scan block of the order kernel (one wavefront)
+constexpr int SCAN_BLOCK = 64;
 // threads per block of the bead kernels
 constexpr int BEAD_BLOCK = 256;
 
@@ -111,5 +111,9 @@ void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const 
 void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, float dmin2,
                      unsigned long long *count);
 void launch_noise(hipStream_t s, const KParams &p, uint64_t step, float4 *out);
+void launch_observables(hipStream_t s, int N, double R_n, double nod, double a, int n_b, const float4 *pos,
+                        double *out);
+void launch_sd_cp(hipStream_t s, const float4 *pos, int start, int end, int s_max, double *sd, double *cp);
+void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm);
 
 } // namespace ccpmi
