@@ -178,15 +178,62 @@ int ccp_contact_map(ccp_sim *h, uint32_t *map, size_t n_bins2, uint32_t *n_sampl
  * 64 bins). */
 int ccp_observables(ccp_sim *h, double scalars[4], double *rcd, size_t n_bins);
 
-/* Mean distance d(s) and contact probability P(s) (d < 2) over the bead
- * pairs (p, p+s) of chain `chain`, s = 1..s_max (calc_sd_cp,
- * chrana.cu:13-45, for one chromosome copy, without ccp's factor 1000). */
-int ccp_distance_contact(ccp_sim *h, uint32_t chain, uint32_t s_max, double *sd, double *cp);
+/* ccp's mean distance d(s) and contact probability P(s) x 1000 (d < 2) of
+ * chromosome `chromosome`, s = 1..n (calc_sd_cp, chrana.cu:13-45, with the
+ * ranges of chrana.cu:93-97,484-491): FP32 sums as ccp. When N = 30386 the
+ * chromosome is 0..5 of ccp's CHR_LIM, averaged with its homologue, and n
+ * is half its length; otherwise chromosome 0 is beads [0, N) and n = N/2. */
+int ccp_distance_contact(ccp_sim *h, uint32_t chromosome, float *sd, float *cp, size_t n);
 
 /* ccp's per-frame contact map of the current positions (calc_cm,
  * chrana.cu:48-84): cms (cms+1)/2 floats of the packed lower triangle, with
  * cms = N/4 (N/8 when N = 30386, averaging the two chromosome copies). */
 int ccp_frame_contact_map(ccp_sim *h, float *cm, size_t lcm);
+
+/* ---- ccp's analysis (chrana, chrana.cuh:20-95; driver ccp-analyze.cu) ----
+ * An analysis handle reads the current state of a simulation handle frame by
+ * frame: ccp's per-frame observables (dcm, rg2, nop, ncf, radial density in
+ * 64 bins) as time series, d(s)/P(s) of each chromosome and the packed
+ * 4-bead contact map as per-simulation sums on the device. Per simulation
+ * it writes analysis-XXX.dat; at the end analysis-fin.dat and
+ * contact-map.bin, in ccp's layouts. */
+typedef struct ccp_analysis ccp_analysis;
+
+/* chrana::chrana (chrana.cu:88-120). Fails if the contact map of N would
+ * exceed 2^31 pixels. The analysis keeps a pointer to h. */
+int ccp_analysis_create(ccp_sim *h, ccp_analysis **out);
+void ccp_analysis_destroy(ccp_analysis *a);
+
+/* chrana::calc_observables (chrana.cu:372-512) on the positions and time
+ * of the simulation handle. */
+int ccp_analysis_add_frame(ccp_analysis *a);
+
+/* chrana::add_trajectory_file (chrana.cu:136-152): read frames_per_file
+ * .trr frames (chrdat::read_frame_bin, chrdat.cu:204-226) into the
+ * simulation handle, adding each. */
+int ccp_analysis_add_trajectory_file(ccp_analysis *a, const char *path);
+
+/* calc_last_is_stat + save_last_is_stat + clear_is_var
+ * (chrana.cu:154-283): statistics of the frames added since the last call,
+ * written to path (analysis-XXX.dat layout). */
+int ccp_analysis_end_simulation(ccp_analysis *a, const char *path);
+
+/* calc_cs_final_stat + save_cs_final_stat + save_cm_avg_bin
+ * (chrana.cu:286-369): combined statistics over the simulations ended so
+ * far, written to fin_path (analysis-fin.dat) and cm_path (contact-map.bin:
+ * lcm float64 averages of the packed lower triangle). */
+int ccp_analysis_finish(ccp_analysis *a, const char *fin_path, const char *cm_path);
+
+/* Error message of the last failed call on a. */
+const char *ccp_analysis_last_error(const ccp_analysis *a);
+
+/* ccp's time-series statistics of n float values (calc_stats for tdstat,
+ * sostat.cu:141-183): out = {avg, var, sem, f_n_b, i_t, thermalised}. */
+int ccp_series_stats(const float *v, size_t n, double out[6]);
+
+/* Inverse-variance weighted combination of n (avg, sem) results
+ * (sostat.cu:186-211): out = {avg, var, sem}. */
+int ccp_weighted_stats(const double *avg, const double *sem, size_t n, double out[3]);
 
 /* Average device time per launch (ms) of each kernel of the step over
  * n_steps steps, measured with HIP events on the handle's stream:

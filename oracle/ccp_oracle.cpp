@@ -572,12 +572,11 @@ int orc_run(const orc_params *p, const uint8_t *flags, const int32_t *chain_off,
 }
 
 int orc_contacts(const float *r, int32_t n, float rc, int32_t bin_size, int32_t n_bins, uint32_t *map) {
-  float rc2 = rc * rc;
   for (int i = 0; i < n; ++i)
     for (int j = i + 1; j < n; ++j) {
       float dx = r[3 * i] - r[3 * j], dy = r[3 * i + 1] - r[3 * j + 1], dz = r[3 * i + 2] - r[3 * j + 2];
       float d2 = (dx * dx + dy * dy) + dz * dz;
-      if (d2 < rc2) {
+      if (sqrtf(d2) < rc) {
         uint32_t bi = (uint32_t)(i / bin_size), bj = (uint32_t)(j / bin_size);
         if (bi < (uint32_t)n_bins && bj < (uint32_t)n_bins) {
           map[(size_t)bi * n_bins + bj] += 1;
@@ -589,7 +588,6 @@ int orc_contacts(const float *r, int32_t n, float rc, int32_t bin_size, int32_t 
 }
 
 int orc_contacts_grid(const float *r, int32_t n, float rc, int32_t bin_size, int32_t n_bins, uint32_t *map) {
-  float rc2 = rc * rc;
   float lo[3] = {r[0], r[1], r[2]}, hi[3] = {r[0], r[1], r[2]};
   for (int i = 0; i < n; ++i)
     for (int a = 0; a < 3; ++a) {
@@ -610,7 +608,7 @@ int orc_contacts_grid(const float *r, int32_t n, float rc, int32_t bin_size, int
             if (j <= i) continue;
             float dx = r[3 * i] - r[3 * j], dy = r[3 * i + 1] - r[3 * j + 1], dz = r[3 * i + 2] - r[3 * j + 2];
             float d2 = (dx * dx + dy * dy) + dz * dz;
-            if (d2 < rc2) {
+            if (sqrtf(d2) < rc) {
               uint32_t bi = (uint32_t)(i / bin_size), bj = (uint32_t)(j / bin_size);
               if (bi < (uint32_t)n_bins && bj < (uint32_t)n_bins) {
                 map[(size_t)bi * n_bins + bj] += 1;
@@ -672,21 +670,31 @@ int orc_observables(const double *r, const uint8_t *flags, int32_t n, double R_n
   return 0;
 }
 
-int orc_sd_cp(const double *r, int32_t start, int32_t end, int32_t s_max, double *sd, double *cp) {
-  for (int s = 1; s <= s_max; ++s) {
-    double sum = 0, cnt = 0;
-    const int m = end - start - s;
-    for (int p = start; p < end - s; ++p) {
-      double dx = r[3 * (p + s)] - r[3 * p], dy = r[3 * (p + s) + 1] - r[3 * p + 1],
-             dz = r[3 * (p + s) + 2] - r[3 * p + 2];
-      double d = std::sqrt(dx * dx + dy * dy + dz * dz);
-      sum += d;
-      if (d < 2.0) cnt += 1.0;
+int orc_sd_cp(const float *r, int32_t n, int32_t chromosome, float *sd, float *cp) {
+  static const uint32_t lim[] = {0, 2284, 4599, 6687, 9338, 12508, 15193, 17477, 19792, 21880, 24531, 27701, 30386};
+  const uint32_t N = (uint32_t)n, N_def = 30386;
+  if (chromosome < 0 || chromosome >= 6 || (N != N_def && chromosome != 0)) return -1;
+  const uint32_t lsdcp = (N == N_def) ? (lim[chromosome + 1] - lim[chromosome]) / 2 : N / 2;
+  const uint32_t ncc = (N == N_def) ? 2 : 1;
+  for (uint32_t ia = 0; ia < lsdcp; ++ia) {
+    const uint32_t s = ia + 1;
+    uint32_t i_s = lim[chromosome], i_e = (N == N_def) ? lim[chromosome + 1] : N;
+    float a = 0.0f, c = 0.0f;
+    for (uint32_t k = 0; k < ncc; ++k) {
+      for (uint32_t p = i_s; p < i_e - s; ++p) {
+        float dx = r[3 * (p + s)] - r[3 * p], dy = r[3 * (p + s) + 1] - r[3 * p + 1],
+              dz = r[3 * (p + s) + 2] - r[3 * p + 2];
+        float d = sqrtf((dx * dx + dy * dy) + dz * dz);
+        a += d;
+        if (d < 2.0f) c += 1000.0f;
+      }
+      i_s += N / ncc;
+      i_e += N / ncc;
     }
-    sd[s - 1] = m > 0 ? sum / m : 0.0;
-    cp[s - 1] = m > 0 ? cnt / m : 0.0;
+    sd[ia] = a / (float)(ncc * (i_e - i_s - s));
+    cp[ia] = c / (float)(ncc * (i_e - i_s - s));
   }
-  return 0;
+  return (int)lsdcp;
 }
 
 int orc_ccp_contact_map(const float *r, int32_t n, float *cm) {
@@ -697,8 +705,10 @@ int orc_ccp_contact_map(const float *r, int32_t n, float *cm) {
   uint32_t ncc = (N == N_def) ? 2 : 1;
   for (uint32_t ia = 0; ia < lcm; ++ia) {
     float acc = 0.0f;
-    uint32_t ixp = (uint32_t)(sqrtf((float)(2.0 * ia + 0.25)) - 0.5);
-    uint32_t iyp = (uint32_t)(ia - ixp * (ixp + 1.0) / 2.0);
+    // ccp's pixel coordinates, clamped to the map where the FP32 root of a
+    // large index would round past the last row
+    uint32_t ixp = std::min((uint32_t)(sqrtf((float)(2.0 * ia + 0.25)) - 0.5), cms - 1);
+    uint32_t iyp = std::min((uint32_t)(ia - ixp * (ixp + 1.0) / 2.0), cms - 1);
     uint32_t sx = ixp * px, ex = sx + px, sy = iyp * px, ey = sy + px;
     for (uint32_t c = 0; c < ncc; ++c) {
       for (uint32_t a = sx; a < ex; ++a)

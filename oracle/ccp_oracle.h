@@ -104,7 +104,8 @@ int orc_run(const orc_params *p, const uint8_t *flags, const int32_t *chain_off,
 
 /* Binned contact counts of ordered pairs i != j with |r_i - r_j| < rc,
  * accumulated into map[bin(i) * n_bins + bin(j)], bin(i) = i / bin_size.
- * Distances use FP32 with no fused operations: d2 = (dx*dx + dy*dy) + dz*dz. */
+ * Distances use FP32 with no fused operations, and ccp's contact test
+ * (calc_cm, chrana.cu:73-76): sqrtf((dx*dx + dy*dy) + dz*dz) < rc. */
 int orc_contacts(const float *r, int32_t n, float rc, int32_t bin_size,
                  int32_t n_bins, uint32_t *map);
 
@@ -126,10 +127,13 @@ int orc_ccp_contact_map(const float *r, int32_t n, float *cm);
 int orc_observables(const double *r, const uint8_t *flags, int32_t n, double R_n, double nod, int32_t n_b,
                     double out[4], double *rcd);
 
-/* Mean distance d(s) and contact probability P(s) (d < 2) over the pairs
- * (p, p+s) of beads [start, end), s = 1..s_max (calc_sd_cp,
- * chrana.cu:13-45, one copy, without ccp's factor 1000). */
-int orc_sd_cp(const double *r, int32_t start, int32_t end, int32_t s_max, double *sd, double *cp);
+/* ccp's mean distance d(s) and contact probability P(s) x 1000 of one
+ * chromosome, s = 1..lsdcp (calc_sd_cp, chrana.cu:13-45, ranges of
+ * chrana.cu:93-97,484-491): FP32 sums over p of |r[p+s] - r[p]| and of 1000
+ * per d < 2, both copies when n = 30386 (chromosome 0..5 of CHR_LIM, lsdcp =
+ * half its length), else chromosome 0 = beads [0, n), lsdcp = n/2.
+ * Returns lsdcp, or -1 for a chromosome that does not exist. */
+int orc_sd_cp(const float *r, int32_t n, int32_t chromosome, float *sd, float *cp);
 
 #ifdef __cplusplus
 }

@@ -65,7 +65,7 @@ def lib():
         L.orc_contacts_grid.argtypes = [f32p, ctypes.c_int32, ctypes.c_float, ctypes.c_int32, ctypes.c_int32, u32p]
         L.orc_observables.argtypes = [f64p, P(ctypes.c_uint8), ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int32, f64p, f64p]
-        L.orc_sd_cp.argtypes = [f64p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, f64p, f64p]
+        L.orc_sd_cp.argtypes = [f32p, ctypes.c_int32, ctypes.c_int32, f32p, f32p]
         L.orc_ccp_contact_map.argtypes = [f32p, ctypes.c_int32, f32p]
         _lib = L
     return _lib
@@ -200,8 +200,14 @@ def observables(r, flags, R_n, nod=None, n_b=64):
     return out, rcd
 
 
-def sd_cp(r, start, end, s_max):
-    r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)
-    sd, cp = np.zeros(s_max), np.zeros(s_max)
-    lib().orc_sd_cp(_p(r, ctypes.c_double), start, end, s_max, _p(sd, ctypes.c_double), _p(cp, ctypes.c_double))
+def sd_cp(r, chromosome=0):
+    """ccp's d(s) and 1000 P(s) of one chromosome (float32 arrays of lsdcp)."""
+    r = np.ascontiguousarray(r, dtype=np.float32).reshape(-1, 3)
+    n = r.shape[0]
+    lim = [0, 2284, 4599, 6687, 9338, 12508, 15193, 17477, 19792, 21880, 24531, 27701, 30386]
+    m = (lim[chromosome + 1] - lim[chromosome]) // 2 if n == 30386 and chromosome < 6 else n // 2
+    sd, cp = np.zeros(m, np.float32), np.zeros(m, np.float32)
+    got = lib().orc_sd_cp(_p(r, ctypes.c_float), n, chromosome, _p(sd, ctypes.c_float), _p(cp, ctypes.c_float))
+    if got != m:
+        raise ValueError("no such chromosome")
     return sd, cp

@@ -1,6 +1,7 @@
 """GPU parity of the analysis kernels (ccp's chrana.cu per-frame work) with
-the oracle: frame observables and radial density, d(s)/P(s), and ccp's
-packed 4-bead contact map including its two-copy average at N = 30386."""
+the oracle: frame observables and radial density (FP64 sums, tolerance in
+the test), d(s)/P(s) and ccp's packed 4-bead contact map (ccp's FP32
+arithmetic, bit-exact), including the two-copy averages at N = 30386."""
 import os
 
 import numpy as np
@@ -49,24 +50,34 @@ def test_observables_match_oracle(N, n_b):
     sim.close()
 
 
-def test_distance_contact_match_oracle():
-    step("d(s), P(s)")
-    off = [0, 700, 2048]
+def test_distance_contact_bit_exact():
+    step("d(s), P(s) N=2048")
     N = 2048
     R_n = radius_for(N, 0.2)
-    r = walk(N, R_n, seed=3, chain_off=off)
-    sim = sim_with(N, R_n, r, np.zeros(N, np.uint8), chain_lengths="700,1348")
-    r32 = sim.positions.astype(np.float64)
-    for c, s_max in ((0, 699), (1, 300), (1, 1)):
-        sd, cp = sim.distance_contact(c, s_max)
-        wsd, wcp = po.sd_cp(r32, off[c], off[c + 1], s_max)
-        assert np.allclose(sd, wsd, rtol=1e-12, atol=0)
-        assert np.array_equal(cp, wcp)
-        assert cp[0] == 1.0  # bonded neighbours of a unit-step walk
-    with pytest.raises(tfm_amd.CcpError):
-        sim.distance_contact(0, 700)
-    with pytest.raises(tfm_amd.CcpError):
-        sim.distance_contact(2, 10)
+    sim = sim_with(N, R_n, walk(N, R_n, seed=3, chain_off=[0, 700, N]), np.zeros(N, np.uint8),
+                   chain_lengths="700,1348")
+    sd, cp = sim.distance_contact(0)
+    wsd, wcp = po.sd_cp(sim.positions, 0)
+    assert sd.shape == (1024,)
+    assert np.array_equal(sd, wsd) and np.array_equal(cp, wcp)
+    with pytest.raises(tfm_amd.CcpError, match="no such chromosome"):
+        sim.distance_contact(1)
+    sim.close()
+
+
+def test_distance_contact_two_copies():
+    step("d(s), P(s) N=30386")
+    N = 30386
+    R_n = radius_for(N, 0.2)
+    sim = tfm_amd.Simulation({"nucleus_radius": repr(float(R_n)), "sequence_file": os.path.join(HERE, "golden",
+                              "sequence.txt"), "seed": 4, "number_of_lbs": 0})
+    sim.positions = walk(N, R_n, seed=21, chain_off=CHR_LIM).astype(np.float32)
+    for c in range(6):
+        sd, cp = sim.distance_contact(c)
+        wsd, wcp = po.sd_cp(sim.positions, c)
+        assert sd.shape == ((CHR_LIM[c + 1] - CHR_LIM[c]) // 2,)
+        assert np.array_equal(sd, wsd) and np.array_equal(cp, wcp), c
+        assert cp[0] == 1000.0
     sim.close()
 
 

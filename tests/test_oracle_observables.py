@@ -12,11 +12,33 @@ def test_straight_chain_distance_and_contacts():
     N = 40
     r = np.zeros((N, 3))
     r[:, 0] = np.arange(N) * 1.0
-    sd, cp = po.sd_cp(r, 0, N, 20)
-    assert np.allclose(sd, np.arange(1, 21))
-    assert cp[0] == 1.0 and np.all(cp[1:] == 0.0)
-    sd, cp = po.sd_cp(r, 10, 30, 5)  # a sub-chain
-    assert np.allclose(sd, np.arange(1, 6))
+    sd, cp = po.sd_cp(r)
+    assert sd.shape == (20,) and sd.dtype == np.float32
+    assert np.array_equal(sd, np.arange(1, 21, dtype=np.float32))
+    assert cp[0] == 1000.0 and np.all(cp[1:] == 0.0)  # d = 2 is not a contact
+    with pytest.raises(ValueError):
+        po.sd_cp(r, 1)  # one chromosome unless N = 30386
+
+
+LIM = [0, 2284, 4599, 6687, 9338, 12508, 15193, 17477, 19792, 21880, 24531, 27701, 30386]
+
+
+@pytest.mark.parametrize("c", [0, 2, 5])
+def test_two_copy_average_matches_brute_force(c):
+    # N = 30386: chromosome c and its homologue c + 6 are averaged; s runs to
+    # half the chromosome length
+    rng = np.random.default_rng(c)
+    v = rng.normal(size=(30386, 3))
+    r = np.cumsum(v / np.linalg.norm(v, axis=1, keepdims=True), axis=0).astype(np.float32)
+    sd, cp = po.sd_cp(r, c)
+    L = LIM[c + 1] - LIM[c]
+    assert sd.shape == (L // 2,)
+    r64 = r.astype(np.float64)
+    for s in (1, 2, 7, 100, L // 2):
+        d = np.concatenate([np.linalg.norm(r64[b + s:b + L] - r64[b:b + L - s], axis=1)
+                            for b in (LIM[c], LIM[c + 6])])
+        assert sd[s - 1] == pytest.approx(d.mean(), rel=2e-5)
+        assert abs(cp[s - 1] - 1000 * (d < 2).mean()) <= 1000 / len(d) + 1e-3
 
 
 def test_shell_and_radial_bond_order():

@@ -1,4 +1,5 @@
-"""Build recipe for libccpmi.so (gfx950) and the ccp-perform drop-in CLI.
+"""Build recipe for libccpmi.so (gfx950) and the ccp-perform / ccp-analyze
+drop-in executables.
 
 Everything is compiled in-tree with hipcc so the built files travel with the
 source tree.
@@ -38,7 +39,7 @@ def build_lib():
 
 def build_cli():
     os.makedirs(BIN, exist_ok=True)
-    for name, src in (("ccp-perform", "ccp_perform.cpp"),):
+    for name, src in (("ccp-perform", "ccp_perform.cpp"), ("ccp-analyze", "ccp_analyze.cpp")):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue

@@ -27,7 +27,9 @@ EXPORTS = (
     "ccp_get_types", "ccp_set_types", "ccp_get_flags", "ccp_set_flags", "ccp_get_lbs", "ccp_set_lbs",
     "ccp_get_chains", "ccp_compute_forces", "ccp_cell_list", "ccp_noise", "ccp_count_overlaps",
     "ccp_contact_sample", "ccp_contact_reset", "ccp_contact_map", "ccp_time_kernels", "ccp_observables",
-    "ccp_distance_contact", "ccp_frame_contact_map",
+    "ccp_distance_contact", "ccp_frame_contact_map", "ccp_analysis_create", "ccp_analysis_destroy",
+    "ccp_analysis_add_frame", "ccp_analysis_add_trajectory_file", "ccp_analysis_end_simulation",
+    "ccp_analysis_finish", "ccp_analysis_last_error", "ccp_series_stats", "ccp_weighted_stats",
 )
 
 
@@ -47,6 +49,9 @@ class Info(ctypes.Structure):
 
 
 LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
+
+# ccp's chromosome limits at N = 30386 (chrdat.cuh:27-29)
+_CHR_LIM = (0, 2284, 4599, 6687, 9338, 12508, 15193, 17477, 19792, 21880, 24531, 27701, 30386)
 
 _lib = None
 
@@ -95,8 +100,18 @@ def lib():
         L.ccp_time_kernels.argtypes = [vp, ctypes.c_uint32, f32p]
         f64p = P(ctypes.c_double)
         L.ccp_observables.argtypes = [vp, f64p, f64p, sz]
-        L.ccp_distance_contact.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, f64p, f64p]
+        L.ccp_distance_contact.argtypes = [vp, ctypes.c_uint32, f32p, f32p, sz]
         L.ccp_frame_contact_map.argtypes = [vp, f32p, sz]
+        L.ccp_analysis_create.argtypes = [vp, P(vp)]
+        L.ccp_analysis_destroy.argtypes = [vp]
+        L.ccp_analysis_add_frame.argtypes = [vp]
+        L.ccp_analysis_add_trajectory_file.argtypes = [vp, ctypes.c_char_p]
+        L.ccp_analysis_end_simulation.argtypes = [vp, ctypes.c_char_p]
+        L.ccp_analysis_finish.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
+        L.ccp_analysis_last_error.argtypes = [vp]
+        L.ccp_analysis_last_error.restype = ctypes.c_char_p
+        L.ccp_series_stats.argtypes = [f32p, sz, f64p]
+        L.ccp_weighted_stats.argtypes = [f64p, f64p, sz, f64p]
         _lib = L
     return _lib
 
@@ -299,10 +314,16 @@ class Simulation:
         self._check(lib().ccp_observables(self.h, _ptr(sc, ctypes.c_double), _ptr(rcd, ctypes.c_double), n_bins))
         return sc, rcd
 
-    def distance_contact(self, chain, s_max):
-        sd, cp = np.zeros(s_max), np.zeros(s_max)
-        self._check(lib().ccp_distance_contact(self.h, chain, s_max, _ptr(sd, ctypes.c_double),
-                                               _ptr(cp, ctypes.c_double)))
+    def distance_contact(self, chromosome=0):
+        """ccp's d(s) and 1000 P(s) of one chromosome (float32, s = 1..n)."""
+        N = self.N
+        if N == 30386 and chromosome < 6:
+            n = (_CHR_LIM[chromosome + 1] - _CHR_LIM[chromosome]) // 2
+        else:
+            n = N // 2
+        sd, cp = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        self._check(lib().ccp_distance_contact(self.h, chromosome, _ptr(sd, ctypes.c_float),
+                                               _ptr(cp, ctypes.c_float), n))
         return sd, cp
 
     def frame_contact_map(self):
@@ -310,3 +331,60 @@ class Simulation:
         cm = np.zeros(cms * (cms + 1) // 2, np.float32)
         self._check(lib().ccp_frame_contact_map(self.h, _ptr(cm, ctypes.c_float), cm.size))
         return cm
+
+
+class Analysis:
+    """ccp's chrana on the frames of a Simulation: per-frame observables,
+    d(s)/P(s) and the packed contact map, written in ccp's analysis files."""
+
+    def __init__(self, sim):
+        self.sim = sim
+        h = ctypes.c_void_p()
+        sim._check(lib().ccp_analysis_create(sim.h, ctypes.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            lib().ccp_analysis_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CcpError(rc, lib().ccp_analysis_last_error(self.h).decode())
+
+    def add_frame(self):
+        self._check(lib().ccp_analysis_add_frame(self.h))
+
+    def add_trajectory_file(self, path):
+        self._check(lib().ccp_analysis_add_trajectory_file(self.h, os.fsencode(path)))
+
+    def end_simulation(self, path):
+        self._check(lib().ccp_analysis_end_simulation(self.h, os.fsencode(path)))
+
+    def finish(self, fin_path, cm_path):
+        self._check(lib().ccp_analysis_finish(self.h, os.fsencode(fin_path), os.fsencode(cm_path)))
+
+
+def series_stats(v):
+    """ccp's time-series statistics: (avg, var, sem, f_n_b, i_t, thermalised)."""
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.zeros(6)
+    rc = lib().ccp_series_stats(_ptr(v, ctypes.c_float), v.size, _ptr(out, ctypes.c_double))
+    if rc:
+        raise CcpError(rc, "bad series")
+    return out
+
+
+def weighted_stats(avg, sem):
+    """Inverse-variance weighted combination: (avg, var, sem)."""
+    avg = np.ascontiguousarray(avg, np.float64)
+    sem = np.ascontiguousarray(sem, np.float64)
+    out = np.zeros(3)
+    rc = lib().ccp_weighted_stats(_ptr(avg, ctypes.c_double), _ptr(sem, ctypes.c_double), avg.size,
+                                  _ptr(out, ctypes.c_double))
+    if rc:
+        raise CcpError(rc, "bad input")
+    return out

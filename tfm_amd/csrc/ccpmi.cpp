@@ -6,6 +6,7 @@
 
 #include "ccpmi.h"
 #include "kernels.hpp"
+#include "stats.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -312,6 +313,20 @@ struct ccp_sim {
   void generate_initial_condition();
   // -------------------------------------------------------------- files
   void write_frame_bin(std::ofstream &f);
+  // -------------------------------------------------------------- analysis
+  // bead range of ccp's chromosome c for d(s)/P(s) (chrana.cu:93-97,484-491)
+  bool sd_cp_range(uint32_t c, uint32_t &i_s, uint32_t &i_e) const {
+    if (N == N_DEF) {
+      if (c >= 6) return false;
+      i_s = CHR_LIM[c];
+      i_e = CHR_LIM[c + 1];
+      return true;
+    }
+    if (c != 0) return false;
+    i_s = 0;
+    i_e = N;
+    return true;
+  }
 };
 
 void ccp_sim::configure(const ParMap &par) {
@@ -504,7 +519,7 @@ void ccp_sim::fill_kparams() {
   k.fcap2 = fcap * fcap;
   k.seed_lo = (uint32_t)seed;
   k.seed_hi = (uint32_t)(seed >> 32);
-  k.contact_rc2 = contact_rc * contact_rc;
+  k.contact_rc = contact_rc;
   k.contact_bin_size = (int)cbin_size;
   k.contact_bins = (int)cbins;
 }
@@ -1490,17 +1505,18 @@ int ccp_observables(ccp_sim *h, double scalars[4], double *rcd, size_t n_bins) {
   });
 }
 
-int ccp_distance_contact(ccp_sim *h, uint32_t chain, uint32_t s_max, double *sd, double *cp) {
+int ccp_distance_contact(ccp_sim *h, uint32_t chromosome, float *sd, float *cp, size_t n) {
   return guarded(h, [&] {
-    need(sd && cp && chain + 1 < h->chain_off.size(), "bad chain");
-    const uint32_t b = h->chain_off[chain], e = h->chain_off[chain + 1];
-    need(s_max >= 1 && s_max < e - b, "s_max out of range");
-    DevBuf<double> out;
-    out.alloc(2 * (size_t)s_max);
-    launch_sd_cp(h->stream, h->r.p, (int)b, (int)e, (int)s_max, out.p, out.p + s_max);
+    uint32_t i_s = 0, i_e = 0;
+    need(sd && cp && h->sd_cp_range(chromosome, i_s, i_e), "no such chromosome");
+    const uint32_t lsdcp = (i_e - i_s) / 2;
+    need(n == lsdcp, "distance/contact size mismatch");
+    DevBuf<float> out;
+    out.alloc(2 * (size_t)lsdcp);
+    launch_sd_cp(h->stream, h->r.p, h->N, lsdcp, i_s, i_e, out.p, out.p + lsdcp, nullptr, nullptr);
     hip_check(hipGetLastError());
-    hip_check(hipMemcpyAsync(sd, out.p, s_max * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    hip_check(hipMemcpyAsync(cp, out.p + s_max, s_max * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipMemcpyAsync(sd, out.p, lsdcp * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipMemcpyAsync(cp, out.p + lsdcp, lsdcp * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     hip_check(hipStreamSynchronize(h->stream));
     out.free_();
   });
@@ -1512,7 +1528,7 @@ int ccp_frame_contact_map(ccp_sim *h, float *cm, size_t lcm) {
     need(cm && lcm == (size_t)cms * (cms + 1) / 2, "contact map size mismatch");
     DevBuf<float> out;
     out.alloc(lcm);
-    launch_ccp_cm(h->stream, h->N, (uint32_t)lcm, h->r.p, out.p);
+    launch_ccp_cm(h->stream, h->N, (uint32_t)lcm, h->r.p, out.p, nullptr);
     hip_check(hipGetLastError());
     hip_check(hipMemcpyAsync(cm, out.p, lcm * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     hip_check(hipStreamSynchronize(h->stream));
@@ -1584,6 +1600,281 @@ int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]) {
     for (auto &e : ev) (void)hipEventDestroy(e);
     h->check_flags();
   });
+}
+
+// ---------------------------------------------------------------- analysis
+} // extern "C"
+
+// ccp's chrana state (chrana.cuh:56-90): time series of the frame
+// observables on the host; the basic observables (d(s), P(s), contact map)
+// as FP64 sums on the device, closed per simulation into sum avg and
+// sum avg^2 for the combined statistics.
+struct ccp_analysis {
+  static constexpr uint32_t N_B = 64; // radial bins (chrana.cuh:12)
+  static constexpr uint32_t N_OBS = 4 + 3 * N_B;
+  ccp_sim *h = nullptr;
+  int dev = 0;
+  std::string err;
+  uint32_t cms = 0, lcm = 0, n_s = 0, n_dp = 0;
+  uint32_t lsdcp[6] = {}, i_s[6] = {}, i_e[6] = {};
+  std::vector<float> ts[N_OBS];           // dcm, rg2, nop, ncf, rcd[3][64]
+  std::vector<SeriesStat> sv[N_OBS];      // per simulation
+  DevBuf<double> obs;                     // per-frame observable sums
+  DevBuf<double> sdcp[6];                 // sum_sd, sum_cp, m1_sd, m1_cp, m2_sd, m2_cp (lsdcp each)
+  DevBuf<double> cm;                      // sum, m1, m2 (lcm each)
+  ~ccp_analysis() {
+    obs.free_();
+    for (auto &b : sdcp) b.free_();
+    cm.free_();
+  }
+};
+
+namespace {
+
+template <class F> int aguarded(ccp_analysis *a, F &&fn) {
+  if (!a) return CCP_ERR_ARG;
+  try {
+    hip_check(hipSetDevice(a->h->dev));
+    fn();
+    return CCP_OK;
+  } catch (const Error &e) {
+    a->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    a->err = e.what();
+    return CCP_ERR_ARG;
+  }
+}
+
+// "avg sqrt(var) sem" columns of ccp's statistics lines (sostat.cu:21-47)
+std::string stat3(const SeriesStat &s) {
+  return cnfs(s.avg, 12, ' ', 6) + cnfs(std::sqrt(s.var), 12, ' ', 6) + cnfs(s.sem, 12, ' ', 6);
+}
+
+void add_frame(ccp_analysis *a) {
+  ccp_sim *h = a->h;
+  constexpr uint32_t nb = ccp_analysis::N_B;
+  const size_t n_acc = 6 + 3 * nb;
+  hip_check(hipMemsetAsync(a->obs.p, 0, n_acc * sizeof(double), h->stream));
+  launch_observables(h->stream, (int)h->N, (double)h->R_n, (double)h->nod, (double)RCO / 2.0, (int)nb, h->r.p,
+                     a->obs.p);
+  for (uint32_t c = 0; c < 6; ++c) {
+    const uint32_t L = a->lsdcp[c];
+    if (!L) continue;
+    double *b = a->sdcp[c].p;
+    launch_sd_cp(h->stream, h->r.p, h->N, L, a->i_s[c], a->i_e[c], nullptr, nullptr, b, b + L);
+  }
+  launch_ccp_cm(h->stream, h->N, a->lcm, h->r.p, nullptr, a->cm.p);
+  hip_check(hipGetLastError());
+  std::vector<double> acc(n_acc);
+  hip_check(hipMemcpyAsync(acc.data(), a->obs.p, n_acc * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  hip_check(hipStreamSynchronize(h->stream));
+  // the series hold FP32 values, as ccp's (FP64 sums here, FP32 in ccp)
+  const double n = h->N, cx = acc[0] / n, cy = acc[1] / n, cz = acc[2] / n;
+  a->ts[0].push_back((float)std::sqrt(cx * cx + cy * cy + cz * cz));
+  a->ts[1].push_back((float)(acc[3] / n - (cx * cx + cy * cy + cz * cz)));
+  a->ts[2].push_back((float)(h->N > 1 ? acc[4] / (n - 1.0) : 0.0));
+  a->ts[3].push_back((float)(acc[5] / n));
+  const double shell = 4.0 / 3.0 * M_PI * (double)h->R_n * h->R_n * h->R_n / nb;
+  for (uint32_t k = 0; k < 3 * nb; ++k) a->ts[4 + k].push_back((float)(acc[6 + k] / shell));
+  ++a->n_dp;
+}
+
+} // namespace
+
+extern "C" {
+
+int ccp_analysis_create(ccp_sim *h, ccp_analysis **out) {
+  return guarded(h, [&] {
+    need(out != nullptr, "null output");
+    *out = nullptr;
+    std::unique_ptr<ccp_analysis> a(new ccp_analysis());
+    a->h = h;
+    a->dev = h->dev;
+    // chrana::chrana (chrana.cu:88-120)
+    a->cms = (h->N == N_DEF) ? (h->N / 2) / 4 : h->N / 4;
+    need(a->cms >= 1 && a->cms <= 65535, "contact map side out of range");
+    a->lcm = a->cms * (a->cms + 1) / 2;
+    for (uint32_t c = 0; c < 6; ++c) {
+      uint32_t b, e;
+      if (h->sd_cp_range(c, b, e)) {
+        a->i_s[c] = b;
+        a->i_e[c] = e;
+        a->lsdcp[c] = (e - b) / 2;
+      }
+      a->sdcp[c].alloc(6 * (size_t)a->lsdcp[c]);
+      if (a->lsdcp[c]) hip_check(hipMemsetAsync(a->sdcp[c].p, 0, 6 * (size_t)a->lsdcp[c] * 8, h->stream));
+    }
+    a->obs.alloc(6 + 3 * ccp_analysis::N_B);
+    a->cm.alloc(3 * (size_t)a->lcm);
+    hip_check(hipMemsetAsync(a->cm.p, 0, 3 * (size_t)a->lcm * 8, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    *out = a.release();
+  });
+}
+
+void ccp_analysis_destroy(ccp_analysis *a) {
+  if (a) (void)hipSetDevice(a->dev);
+  delete a;
+}
+
+const char *ccp_analysis_last_error(const ccp_analysis *a) { return a ? a->err.c_str() : "null handle"; }
+
+int ccp_analysis_add_frame(ccp_analysis *a) {
+  return aguarded(a, [&] { add_frame(a); });
+}
+
+int ccp_analysis_add_trajectory_file(ccp_analysis *a, const char *path) {
+  return aguarded(a, [&] {
+    need(path != nullptr, "null path");
+    ccp_sim *h = a->h;
+    std::ifstream f(path, std::ios::binary);
+    if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + path);
+    for (uint32_t ffi = 0; ffi < h->fpf; ++ffi) {
+      h->progress((float)(100.0 * ffi / h->fpf));
+      // chrdat::read_frame_bin (chrdat.cu:204-226)
+      uint32_t header[18];
+      f.read(reinterpret_cast<char *>(header), sizeof(header));
+      std::vector<float> r(3 * (size_t)h->N);
+      f.read(reinterpret_cast<char *>(r.data()), 12 * (size_t)h->N);
+      if (f.fail()) throw Error(CCP_ERR_IO, "failed to read frame from binary file");
+      h->i_f = header[14];
+      std::memcpy(&h->t, &header[16], 4);
+      h->hr = r;
+      h->upload_positions();
+      add_frame(a);
+    }
+  });
+}
+
+int ccp_analysis_end_simulation(ccp_analysis *a, const char *path) {
+  return aguarded(a, [&] {
+    need(path != nullptr, "null path");
+    need(a->n_dp > 0, "no frames added");
+    ccp_sim *h = a->h;
+    constexpr uint32_t nb = ccp_analysis::N_B;
+    // calc_last_is_stat (chrana.cu:154-181)
+    for (uint32_t k = 0; k < ccp_analysis::N_OBS; ++k)
+      a->sv[k].push_back(series_stats(a->ts[k].data(), a->ts[k].size()));
+    std::vector<std::vector<double>> sd(6), cp(6);
+    for (uint32_t c = 0; c < 6; ++c) {
+      const uint32_t L = a->lsdcp[c];
+      sd[c].resize(L);
+      cp[c].resize(L);
+      if (!L) continue;
+      hip_check(hipMemcpyAsync(sd[c].data(), a->sdcp[c].p, L * 8, hipMemcpyDeviceToHost, h->stream));
+      hip_check(hipMemcpyAsync(cp[c].data(), a->sdcp[c].p + L, L * 8, hipMemcpyDeviceToHost, h->stream));
+    }
+    hip_check(hipStreamSynchronize(h->stream));
+    // save_last_is_stat (chrana.cu:184-247)
+    std::ofstream f(path);
+    if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + path);
+    f << "         avg   sqrt(var)         sem   f_n_b     i_t ter\n";
+    for (uint32_t k = 0; k < 4; ++k) {
+      const SeriesStat &t = a->sv[k].back();
+      f << stat3(t) << cnfs(t.f_n_b, 8, ' ') << cnfs(t.i_t, 8, ' ') << (t.thm ? " yes" : "  no") << "\n";
+    }
+    f << "\n\n";
+    for (uint32_t ty = 0; ty < 3; ++ty) {
+      f << "         r_b         avg   sqrt(var)         sem\n";
+      f << "    0.000000    0.000000    0.000000    0.000000\n";
+      for (uint32_t b = 0; b < nb; ++b)
+        f << cnfs(h->R_n * std::pow((b + 1.0) / nb, 1.0 / 3), 12, ' ', 6) << stat3(a->sv[4 + ty * nb + b].back())
+          << "\n";
+      f << "\n\n";
+    }
+    for (auto *v : {&sd, &cp})
+      for (uint32_t c = 0; c < 6; ++c) {
+        f << "     s         avg\n";
+        for (uint32_t i = 0; i < a->lsdcp[c]; ++i)
+          f << cnfs(i + 1, 6, ' ') << cnfs((*v)[c][i] / a->n_dp, 12, ' ', 6) << "\n";
+        f << "\n\n";
+      }
+    if (f.fail()) throw Error(CCP_ERR_IO, std::string("failed to write ") + path);
+    // close the basic observables and clear the series (clear_is_var, chrana.cu:250-283)
+    for (uint32_t c = 0; c < 6; ++c) {
+      const uint32_t L = a->lsdcp[c];
+      double *b = a->sdcp[c].p;
+      launch_close_sums(h->stream, b, b + 2 * L, b + 4 * L, L, a->n_dp);
+      launch_close_sums(h->stream, b + L, b + 3 * L, b + 5 * L, L, a->n_dp);
+    }
+    launch_close_sums(h->stream, a->cm.p, a->cm.p + a->lcm, a->cm.p + 2 * (size_t)a->lcm, a->lcm, a->n_dp);
+    hip_check(hipGetLastError());
+    hip_check(hipStreamSynchronize(h->stream));
+    for (auto &t : a->ts) t.clear();
+    a->n_dp = 0;
+    ++a->n_s;
+  });
+}
+
+int ccp_analysis_finish(ccp_analysis *a, const char *fin_path, const char *cm_path) {
+  return aguarded(a, [&] {
+    need(fin_path != nullptr && cm_path != nullptr, "null path");
+    need(a->n_s > 0, "no simulations ended");
+    ccp_sim *h = a->h;
+    constexpr uint32_t nb = ccp_analysis::N_B;
+    // calc_cs_final_stat + save_cs_final_stat (chrana.cu:286-360)
+    std::ofstream f(fin_path);
+    if (!f.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + fin_path);
+    f << "         avg   sqrt(var)         sem\n";
+    for (uint32_t k = 0; k < 4; ++k) f << stat3(weighted_stats(a->sv[k])) << "\n";
+    f << "\n\n";
+    for (uint32_t ty = 0; ty < 3; ++ty) {
+      f << "         r_b         avg   sqrt(var)         sem\n";
+      f << "    0.000000    0.000000    0.000000    0.000000\n";
+      for (uint32_t b = 0; b < nb; ++b)
+        f << cnfs(h->R_n * std::pow((b + 1.0) / nb, 1.0 / 3), 12, ' ', 6)
+          << stat3(weighted_stats(a->sv[4 + ty * nb + b])) << "\n";
+      f << "\n\n";
+    }
+    for (int which = 0; which < 2; ++which)
+      for (uint32_t c = 0; c < 6; ++c) {
+        const uint32_t L = a->lsdcp[c];
+        std::vector<double> m1(L), m2(L);
+        if (L) {
+          hip_check(hipMemcpy(m1.data(), a->sdcp[c].p + (2 + which) * (size_t)L, L * 8, hipMemcpyDeviceToHost));
+          hip_check(hipMemcpy(m2.data(), a->sdcp[c].p + (4 + which) * (size_t)L, L * 8, hipMemcpyDeviceToHost));
+        }
+        f << "     s         avg   sqrt(var)         sem\n";
+        for (uint32_t i = 0; i < L; ++i) f << cnfs(i + 1, 6, ' ') << stat3(plain_stats(m1[i], m2[i], a->n_s)) << "\n";
+        f << "\n\n";
+      }
+    if (f.fail()) throw Error(CCP_ERR_IO, std::string("failed to write ") + fin_path);
+    // save_cm_avg_bin (chrana.cu:363-369): the average of each pixel
+    std::vector<double> m1(a->lcm);
+    hip_check(hipMemcpy(m1.data(), a->cm.p + a->lcm, (size_t)a->lcm * 8, hipMemcpyDeviceToHost));
+    for (auto &v : m1) v = v / a->n_s;
+    std::ofstream b(cm_path, std::ios::binary);
+    if (!b.is_open()) throw Error(CCP_ERR_IO, std::string("unable to open ") + cm_path);
+    b.write(reinterpret_cast<const char *>(m1.data()), (std::streamsize)m1.size() * 8);
+    if (b.fail()) throw Error(CCP_ERR_IO, std::string("failed to write ") + cm_path);
+  });
+}
+
+int ccp_series_stats(const float *v, size_t n, double out[6]) {
+  if (!v || !out || n == 0) return CCP_ERR_ARG;
+  const SeriesStat s = series_stats(v, n);
+  out[0] = s.avg;
+  out[1] = s.var;
+  out[2] = s.sem;
+  out[3] = s.f_n_b;
+  out[4] = s.i_t;
+  out[5] = s.thm ? 1.0 : 0.0;
+  return CCP_OK;
+}
+
+int ccp_weighted_stats(const double *avg, const double *sem, size_t n, double out[3]) {
+  if (!avg || !sem || !out || n == 0) return CCP_ERR_ARG;
+  std::vector<SeriesStat> v(n);
+  for (size_t i = 0; i < n; ++i) {
+    v[i].avg = avg[i];
+    v[i].sem = sem[i];
+  }
+  const SeriesStat s = weighted_stats(v);
+  out[0] = s.avg;
+  out[1] = s.var;
+  out[2] = s.sem;
+  return CCP_OK;
 }
 
 } // extern "C"

@@ -704,7 +704,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4
         const float dx = me.x - o.x, dy2 = me.y - o.y, dz2 = me.z - o.z;
         const float xx = dx * dx, yy = dy2 * dy2, zz = dz2 * dz2;
         const float d2 = (xx + yy) + zz;
-        if (d2 < p.contact_rc2) {
+        if (sqrtf(d2) < p.contact_rc) {
           const uint32_t bj = j / (uint32_t)p.contact_bin_size;
           if (bi < nb && bj < nb) {
             atomicAdd(map + (size_t)bi * nb + bj, 1u);
@@ -802,36 +802,59 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_observables(int N, double R_n, d
     if (s_acc[k] != 0.0) atomicAdd(&out[k], s_acc[k]);
 }
 
-// Mean distance and contact fraction (d < 2) over pairs (p, p+s) of beads
-// [start, end), one thread per separation s (calc_sd_cp, chrana.cu:13-45).
-__global__ __launch_bounds__(BEAD_BLOCK) void k_sd_cp(const float4 *pos, int start, int end, int s_max, double *sd,
-                                                      double *cp) {
-  const int s = blockIdx.x * BEAD_BLOCK + threadIdx.x + 1;
-  if (s > s_max) return;
-  double sum = 0.0, cnt = 0.0;
-  for (int p = start; p < end - s; ++p) {
-    const float4 u = pos[p], v = pos[p + s];
-    const double dx = (double)v.x - u.x, dy = (double)v.y - u.y, dz = (double)v.z - u.z;
-    const double d = sqrt(dx * dx + dy * dy + dz * dz);
-    sum += d;
-    if (d < 2.0) cnt += 1.0;
+// ccp's d(s) and 1000 P(s) of one chromosome (calc_sd_cp, chrana.cu:13-45):
+// one thread per separation s = ia + 1, FP32 sums over the pairs (p, p+s) of
+// beads [i_s, i_e) and, when N = 30386, of the homologue N/2 further on.
+// Writes the frame's values to sd/cp and, when given, adds them to the FP64
+// per-simulation sums as ccp's host loop does (chrana.cu:496-502).
+__global__ __launch_bounds__(BEAD_BLOCK) void k_sd_cp(const float4 *pos, uint32_t N, uint32_t lsdcp, uint32_t i_s,
+                                                      uint32_t i_e, float *sd, float *cp, double *sum_sd,
+                                                      double *sum_cp) {
+#pragma clang fp contract(off)
+  const uint32_t ia = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (ia >= lsdcp) return;
+  const uint32_t s = ia + 1, ncc = (N == 30386u) ? 2u : 1u;
+  float a = 0.0f, c = 0.0f;
+  for (uint32_t k = 0; k < ncc; ++k) {
+    for (uint32_t p = i_s; p < i_e - s; ++p) {
+      const float4 u = pos[p], v = pos[p + s];
+      const float dx = v.x - u.x, dy = v.y - u.y, dz = v.z - u.z;
+      const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+      const float d = sqrtf((xx + yy) + zz);
+      a += d;
+      if (d < 2.0f) c += 1000.0f;
+    }
+    i_s += N / ncc;
+    i_e += N / ncc;
   }
-  const int m = end - start - s;
-  sd[s - 1] = m > 0 ? sum / m : 0.0;
-  cp[s - 1] = m > 0 ? cnt / m : 0.0;
+  const float m = (float)(ncc * (i_e - i_s - s));
+  a /= m;
+  c /= m;
+  if (sd) {
+    sd[ia] = a;
+    cp[ia] = c;
+  }
+  if (sum_sd) {
+    sum_sd[ia] += (double)a;
+    sum_cp[ia] += (double)c;
+  }
 }
 
 // ccp's per-frame contact map (calc_cm, chrana.cu:48-84) with its exact
-// float arithmetic: pixel i_a of the packed lower triangle, 4-bead pixels,
+// float arithmetic: pixel ia of the packed lower triangle, 4-bead pixels,
 // contact if |r_p - r_q| < 2, factor 1000, average over the two chromosome
-// copies when N = 30386.
-__global__ __launch_bounds__(BEAD_BLOCK) void k_ccp_cm(uint32_t N, uint32_t lcm, const float4 *pos, float *cm) {
+// copies when N = 30386. Writes the frame's map to cm and, when given, adds
+// it to the FP64 per-simulation sum (chrana.cu:505-511).
+__global__ __launch_bounds__(BEAD_BLOCK) void k_ccp_cm(uint32_t N, uint32_t lcm, const float4 *pos, float *cm,
+                                                       double *sum) {
 #pragma clang fp contract(off)
   const uint32_t ia = blockIdx.x * BEAD_BLOCK + threadIdx.x;
   if (ia >= lcm) return;
-  const uint32_t px = 4, ncc = (N == 30386u) ? 2u : 1u;
-  const uint32_t ixp = (uint32_t)((double)sqrtf((float)(2.0 * ia + 0.25)) - 0.5);
-  const uint32_t iyp = (uint32_t)(ia - ixp * (ixp + 1.0) / 2.0);
+  const uint32_t px = 4, ncc = (N == 30386u) ? 2u : 1u, cms = (N / ncc) / px;
+  // ccp's pixel coordinates; the clamps keep the reads inside the chain
+  // where the FP32 root of a large index would round past the last row
+  const uint32_t ixp = min((uint32_t)((double)sqrtf((float)(2.0 * ia + 0.25)) - 0.5), cms - 1);
+  const uint32_t iyp = min((uint32_t)(ia - ixp * (ixp + 1.0) / 2.0), cms - 1);
   uint32_t sx = ixp * px, ex = sx + px, sy = iyp * px, ey = sy + px;
   float acc = 0.0f;
   for (uint32_t c = 0; c < ncc; ++c) {
@@ -845,7 +868,23 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_ccp_cm(uint32_t N, uint32_t lcm,
       }
     sx += N / ncc; ex += N / ncc; sy += N / ncc; ey += N / ncc;
   }
-  cm[ia] = acc / (float)(ncc * px * px);
+  const float v = acc / (float)(ncc * px * px);
+  if (cm) cm[ia] = v;
+  if (sum) sum[ia] += (double)v;
+}
+
+// Closes one simulation of ccp's basic observables (simobs_b,
+// sostat.cu:48-53,214-235): avg = sum / n_dp, m1 += avg, m2 += avg^2, and
+// the sum restarts. FP64 with no fused operations, as ccp's host loop.
+__global__ __launch_bounds__(BEAD_BLOCK) void k_close_sums(double *sum, double *m1, double *m2, uint32_t n,
+                                                           double n_dp) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const double avg = sum[i] / n_dp;
+  m1[i] += avg;
+  m2[i] += avg * avg;
+  sum[i] = 0.0;
 }
 
 inline unsigned blocks_for(int n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -887,7 +926,7 @@ void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KP
 
 void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map) {
   if (p.N <= 0) return;
-  const int rad = (int)ceilf(sqrtf(p.contact_rc2) / p.gf.csl);
+  const int rad = (int)ceilf(p.contact_rc / p.gf.csl);
   k_contacts<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, map, rad);
 }
 
@@ -909,14 +948,20 @@ void launch_observables(hipStream_t s, int N, double R_n, double nod, double a, 
                                                                                                pos, out);
 }
 
-void launch_sd_cp(hipStream_t s, const float4 *pos, int start, int end, int s_max, double *sd, double *cp) {
-  if (s_max <= 0) return;
-  k_sd_cp<<<blocks_for(s_max, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(pos, start, end, s_max, sd, cp);
+void launch_sd_cp(hipStream_t s, const float4 *pos, uint32_t N, uint32_t lsdcp, uint32_t i_s, uint32_t i_e,
+                  float *sd, float *cp, double *sum_sd, double *sum_cp) {
+  if (lsdcp == 0) return;
+  k_sd_cp<<<blocks_for((int)lsdcp, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(pos, N, lsdcp, i_s, i_e, sd, cp, sum_sd, sum_cp);
 }
 
-void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm) {
+void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm, double *sum) {
   if (lcm == 0) return;
-  k_ccp_cm<<<blocks_for((int)lcm, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(N, lcm, pos, cm);
+  k_ccp_cm<<<blocks_for((int)lcm, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(N, lcm, pos, cm, sum);
+}
+
+void launch_close_sums(hipStream_t s, double *sum, double *m1, double *m2, uint32_t n, double n_dp) {
+  if (n == 0) return;
+  k_close_sums<<<blocks_for((int)n, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(sum, m1, m2, n, n_dp);
 }
 
 } // namespace ccpmi

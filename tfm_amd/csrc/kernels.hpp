@@ -69,7 +69,7 @@ struct KParams {
   float fcap2; // squared force cap of the integrator (0 = off)
   uint32_t seed_lo, seed_hi;
   // contacts
-  float contact_rc2;
+  float contact_rc;
   int contact_bin_size, contact_bins;
 };
 
@@ -113,7 +113,9 @@ void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const 
 void launch_noise(hipStream_t s, const KParams &p, uint64_t step, float4 *out);
 void launch_observables(hipStream_t s, int N, double R_n, double nod, double a, int n_b, const float4 *pos,
                         double *out);
-void launch_sd_cp(hipStream_t s, const float4 *pos, int start, int end, int s_max, double *sd, double *cp);
-void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm);
+void launch_sd_cp(hipStream_t s, const float4 *pos, uint32_t N, uint32_t lsdcp, uint32_t i_s, uint32_t i_e,
+                  float *sd, float *cp, double *sum_sd, double *sum_cp);
+void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm, double *sum);
+void launch_close_sums(hipStream_t s, double *sum, double *m1, double *m2, uint32_t n, double n_dp);
 
 } // namespace ccpmi
