@@ -77,7 +77,25 @@ def parse():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_force_lj_n131072_traffic.txt"))
     ap.add_argument("--param", action="append", default=[], metavar="KEY=VALUE",
                     help="override a simulation parameter (e.g. fine_cell_size=1.25)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain elementwise kernel of known traffic (256 MiB read + 256 MiB "
+                         "written per launch, 10 launches) to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """torch.neg over 2^26 float32: 256 MiB read and 256 MiB written per launch."""
+    import torch
+
+    a = torch.ones(1 << 26, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(10):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"kernel": "neg (elementwise)", "launches": 10,
+                                           "bytes_read_per_launch": 4 << 26, "bytes_written_per_launch": 4 << 26}}),
+          flush=True)
 
 
 def read_traffic(path):
@@ -119,6 +137,9 @@ def cpu_baseline(sim, budget_s):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
               "(independent replicas shard one per GPU with no exchange)", file=sys.stderr)

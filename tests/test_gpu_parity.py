@@ -46,6 +46,12 @@ FORCE_CASES = [
     dict(ff=po.FF_ICG, n_l=0),
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2),
     dict(ff=po.FF_CCP, n_l=20, opening_radius=4.0, bleb_radius=6.0),
+    # one lane per bead (the autotuner's other kernel shape)
+    dict(ff=po.FF_CCP, n_l=40, lanes_per_bead=1),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, lanes_per_bead=1),
+    # pair sum split by cutoff over a fine grid and a het grid
+    dict(ff=po.FF_CCP, n_l=40, fine_cell_size=1.2),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, fine_cell_size=1.25),
 ]
 
 

@@ -5,7 +5,7 @@
 // kernels of kernels.hip, captured into hipGraphs of `graph_steps` steps.
 
 #include "ccpmi.h"
-#include "kernels.hpp"
+#include "sim.hpp"
 #include "stats.hpp"
 
 #include <algorithm>
@@ -24,310 +24,8 @@
 #include <vector>
 
 using namespace ccpmi;
+using namespace ccpmi::host;
 
-namespace {
-
-// ---------------------------------------------------------------- constants
-// ccp model constants (chrdat.cuh:15-24, chrsim.cuh:18-20)
-constexpr float K_B = 0.003356f;
-constexpr float L_0 = 1.0f;
-constexpr float K_E = 128.0f;
-constexpr float K_BEND = 1.0f;
-constexpr float E_P = 0.5f;
-constexpr float RCO = 1.154701f;
-constexpr float ACO = 2.0f;
-constexpr float MDL = 21.10139f;
-constexpr float E_L = 8.0f;
-constexpr float LCO = 0.5f;
-constexpr float DT_DEF = 1.0f / 2048;
-constexpr float MIS = 0.838732f;
-constexpr uint32_t N_DEF = 30386;
-const uint32_t CHR_LIM[13] = {0, 2284, 4599, 6687, 9338, 12508, 15193, 17477, 19792, 21880, 24531, 27701, 30386};
-
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-void hip_check(hipError_t e) {
-  if (e != hipSuccess) throw Error(CCP_ERR_HIP, std::string("hip: ") + hipGetErrorString(e));
-}
-
-// whitespace "key value" pairs; the first occurrence of a key wins (util.cu:72-80)
-struct ParMap : std::map<std::string, std::string> {
-  explicit ParMap(const char *text) {
-    std::istringstream is(text ? text : "");
-    std::string k, v;
-    while (is >> k >> v) insert({k, v});
-  }
-  template <class T> T get(const std::string &key, T def) const {
-    auto it = find(key);
-    if (it == end()) return def;
-    T val{};
-    std::stringstream{it->second} >> val;
-    return val;
-  }
-  bool has(const std::string &k) const { return find(k) != end(); }
-};
-
-// cnfs (util.cuh:91-104): formatted number
-template <class T> std::string cnfs(T num, int len = 0, char fillc = ' ', int prc = 0) {
-  std::stringstream s;
-  if (len > 0) s << std::setw(len);
-  if (fillc != ' ') s << std::setfill(fillc);
-  if (prc > 0) s << std::setprecision(prc) << std::fixed;
-  s << num;
-  return s.str();
-}
-
-// ---------------------------------------------------------------- host Philox stream
-// Uniforms for the initial condition: Philox4x32-10 with the simulation seed
-// as key and counter {n lo, n hi, 0xffffffff, 1 << 16}, disjoint from the
-// step-noise counters {bead, step lo, step hi, attempt < 2^16}.
-struct HostRng {
-  uint32_t k0, k1;
-  uint64_t n = 0;
-  uint32_t buf[4];
-  int pos = 4;
-  explicit HostRng(uint64_t seed) : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)) {}
-  void refill() {
-    uint32_t c[4] = {(uint32_t)n, (uint32_t)(n >> 32), 0xffffffffu, 1u << 16};
-    uint32_t a = k0, b = k1;
-    for (int r = 0; r < 10; ++r) {
-      uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
-      uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ a, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ b;
-      c[1] = (uint32_t)p1;
-      c[3] = (uint32_t)p0;
-      c[0] = n0;
-      c[2] = n2;
-      a += 0x9E3779B9u;
-      b += 0xBB67AE85u;
-    }
-    std::memcpy(buf, c, sizeof(c));
-    ++n;
-    pos = 0;
-  }
-  // uniform in (0, 1] like curandGenerateUniform
-  float uniform() {
-    if (pos == 4) refill();
-    return (float)((buf[pos++] >> 8) + 1u) * (1.0f / 16777216.0f);
-  }
-};
-
-// inverse error function: Winitzki's closed-form start (2008) refined by
-// Newton steps on std::erf
-double erfinv(double x) {
-  if (x <= -1.0) return -INFINITY;
-  if (x >= 1.0) return INFINITY;
-  const double a = 0.147, ln = std::log(1.0 - x * x), t = 2.0 / (M_PI * a) + ln / 2.0;
-  double y = std::copysign(std::sqrt(std::sqrt(t * t - ln / a) - t), x);
-  for (int k = 0; k < 4; ++k) y -= (std::erf(y) - x) / (2.0 / std::sqrt(M_PI) * std::exp(-y * y));
-  return y;
-}
-
-struct V3 {
-  float x, y, z;
-};
-inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
-inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-inline float length(V3 a) { return std::sqrt(dot(a, a)); }
-inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-
-template <class T> struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  void alloc(size_t count) {
-    n = count;
-    hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
-  }
-  void free_() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-  }
-};
-
-} // namespace
-
-// ---------------------------------------------------------------- handle
-struct ccp_sim {
-  // parameters
-  uint32_t N = 0, n_l = 0, fpf = 0, spf = 0, tpb = 0;
-  float T = 0, R_n = 0, R_o = 0, R_b = 0;
-  float noc = 0, boc = 0, nod = 0, bod = 0, d_m = 0;
-  float dt = 0, sd = 0;
-  uint64_t seed = 0;
-  int ff = CCP_FF_CCP;
-  float csl = 0;   // coarse cell side: het grid and ccp-layout export grid
-  float csl_f = 0; // cell side of the all-bead grid of the step (fine when split)
-  bool split = false; // short-range pairs on the fine grid + het-het tail on the het grid
-  std::vector<uint32_t> chain_off;
-  std::string seq_file;
-  int seq_mode = 0; // 0 ccp, 1 het/LAD generator
-  float het_fraction = 0.3f, het_mean = 20.0f, lad_fraction = 0.1f;
-  int lad_ends = 1;
-  float k_b = K_BEND;
-  int bend_all = 0, bond_type = 0;
-  float eps_het = 0.5f, eps_lam = 1.0f;
-  float contact_rc = 0;
-  uint32_t cbins = 0, cbin_size = 0, cinterval = 0;
-  uint32_t graph_steps = 64;
-  float fcap = 0.0f;
-  KParams kp{};
-
-  // state
-  uint32_t i_f = 0;
-  float t = 0.0f;
-  uint64_t step = 0;
-  std::vector<float> hr, hlr;
-  std::vector<uint8_t> hfl;
-  bool lgrid_valid = false, dstep_dirty = true;
-  uint32_t n_samples = 0;
-
-  // device
-  int dev = 0;
-  hipStream_t stream = nullptr;
-  DevBuf<float4> r, er, pstar, lr, lspos, aux;
-  DevBuf<uint32_t> lstart, flags, cmap;
-  DevBuf<uint64_t> dstep;
-  DevBuf<unsigned long long> dcount;
-  // one uniform grid: binning buffers + ordered cell list
-  struct HostGrid {
-    GridGeom g{};
-    DevBuf<uint32_t> count, bs0, bs1, bucket, cstart, soi;
-    DevBuf<float4> spos;
-    int clean = 0;       // which per-block total buffer is zero
-    bool valid = false;  // holds the cell list of the current positions
-    bool allocated = false;
-    uint32_t *bs(int k) { return k ? bs1.p : bs0.p; }
-    void alloc(size_t n_obj) {
-      count.alloc(g.n_c); bs0.alloc(g.nblk); bs1.alloc(g.nblk);
-      bucket.alloc((size_t)g.n_c * g.cap); cstart.alloc(g.n_c + 1); soi.alloc(n_obj); spos.alloc(n_obj);
-      hip_check(hipMemset(count.p, 0, (size_t)g.n_c * 4));
-      hip_check(hipMemset(bs0.p, 0, (size_t)g.nblk * 4));
-      hip_check(hipMemset(bs1.p, 0, (size_t)g.nblk * 4));
-      hip_check(hipMemset(cstart.p, 0, ((size_t)g.n_c + 1) * 4));
-      clean = 0;
-      valid = false;
-      allocated = true;
-    }
-    void free_() {
-      DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &bucket, &cstart, &soi};
-      for (auto *b : u) b->free_();
-      spos.free_();
-      allocated = false;
-    }
-  };
-  HostGrid gf; // fine grid of every bead
-  GridGeom lg{}; // LBS sites on the fine geometry (uses gf's binning buffers)
-  HostGrid gh; // coarse grid of het beads
-  HostGrid gx; // coarse grid of every bead in ccp's geometry (cell-list export)
-  hipGraphExec_t graphs[3][2][2] = {};
-
-  // misc
-  std::string err;
-  ccp_log_fn log_fn = nullptr;
-  void *log_user = nullptr;
-
-  void drop_graphs() {
-    for (auto &gm : graphs)
-      for (auto &gr : gm)
-        for (auto &g : gr)
-          if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-          }
-  }
-
-  ~ccp_sim() {
-    drop_graphs();
-    DevBuf<float4> *f4[] = {&r, &er, &pstar, &lr, &lspos, &aux};
-    for (auto *b : f4) b->free_();
-    DevBuf<uint32_t> *u4[] = {&lstart, &flags, &cmap};
-    for (auto *b : u4) b->free_();
-    gf.free_();
-    gh.free_();
-    gx.free_();
-    dstep.free_();
-    dcount.free_();
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-
-  // -------------------------------------------------------------- logging
-  void record(const std::string &msg) {
-    if (log_fn) {
-      log_fn(msg.c_str(), log_user);
-      return;
-    }
-    time_t now = time(nullptr);
-    char ts[21];
-    strftime(ts, 21, "[%d/%m/%y %H:%M:%S] ", localtime(&now));
-    std::cout << ts << msg << std::endl;
-  }
-  void progress(float pc) {
-    std::string line = "progress: " + cnfs(pc, 5, '0', 1) + "%";
-    if (log_fn) {
-      log_fn(line.c_str(), log_user);
-      return;
-    }
-    std::cout << line << "\r";
-    std::cout.flush();
-  }
-
-  // -------------------------------------------------------------- setup
-  void configure(const ParMap &par);
-  void allocate();
-  void fill_kparams();
-  uint32_t wbits(uint32_t i) const;
-  void upload_positions();
-  void upload_lbs();
-  void download_positions();
-  void check_flags();
-  // -------------------------------------------------------------- step machinery
-  KParams mode_params(int mode) const {
-    KParams k = kp;
-    if (mode == MODE_ICG) k.sd = 0.0f;
-    return k;
-  }
-  void build_grid(HostGrid &G, const GridGeom &geom, const float4 *pos, int n_obj, bool het_only, float4 *out_spos,
-                  uint32_t *out_cstart, uint32_t *out_soi);
-  bool take_overflow();
-  void grow_capacity();
-  void ensure_grids();
-  void invalidate() { gf.valid = gh.valid = gx.valid = false; }
-  void sync_step();
-  void launch_stage(int mode, int stage);
-  void launch_step(int mode) {
-    launch_stage(mode, 1);
-    launch_stage(mode, 2);
-  }
-  void run_steps(uint64_t n, int mode);
-  void run_chunk(uint64_t n, int mode);
-  // -------------------------------------------------------------- initial condition
-  void set_lbs_positions(HostRng &g);
-  void set_particle_types(HostRng &g);
-  void set_particle_positions(HostRng &g);
-  void perform_random_walk(HostRng &g, uint32_t i_s, uint32_t i_e, V3 dir, float mda);
-  uint64_t count_overlaps(float dmin);
-  void generate_initial_condition();
-  // -------------------------------------------------------------- files
-  void write_frame_bin(std::ofstream &f);
-  // -------------------------------------------------------------- analysis
-  // bead range of ccp's chromosome c for d(s)/P(s) (chrana.cu:93-97,484-491)
-  bool sd_cp_range(uint32_t c, uint32_t &i_s, uint32_t &i_e) const {
-    if (N == N_DEF) {
-      if (c >= 6) return false;
-      i_s = CHR_LIM[c];
-      i_e = CHR_LIM[c + 1];
-      return true;
-    }
-    if (c != 0) return false;
-    i_s = 0;
-    i_e = N;
-    return true;
-  }
-};
 
 void ccp_sim::configure(const ParMap &par) {
   // chrdat::chrdat and cngeom (chrdat.cu:13-75), chrsim::chrsim (chrsim.cu:421-439)
@@ -425,6 +123,8 @@ void ccp_sim::configure(const ParMap &par) {
   lg = gf.g;
   fcap = par.get<float>("force_cap", lj ? 0.2f / dt : 0.0f);
   if (!(fcap >= 0.0f)) throw Error(CCP_ERR_PARAM, "force_cap out of range");
+  lanes = par.get<uint32_t>("lanes_per_bead", 3u);
+  if (lanes != 1 && lanes != 3) throw Error(CCP_ERR_PARAM, "lanes_per_bead out of range");
   graph_steps = par.get<uint32_t>("graph_steps", 64u);
   if (!(1 <= graph_steps && graph_steps <= 4096)) throw Error(CCP_ERR_PARAM, "graph_steps out of range");
 
@@ -487,6 +187,7 @@ void ccp_sim::fill_kparams() {
   k.gh = gh.g;
   if (!split) k.gh.n_c = 0;
   k.split = split ? 1 : 0;
+  k.lanes = (int)lanes;
   k.R_n = R_n; k.R_o = R_o; k.R_b = R_b;
   k.noc = noc; k.boc = boc; k.nod = nod; k.bod = bod;
   const bool lj = ff == CCP_FF_LJ;

@@ -1,28 +1,32 @@
-// gfx950 kernels of the ccp Brownian-dynamics step.
+// gfx950 kernels of the ccp Brownian-dynamics step and of ccp's analysis.
 //
 // One stochastic-Heun stage is two launches:
-//   k_force<MODE, STAGE>  one thread per bead in fine-grid order. Short-range
-//                         pairs (every pair type, cutoff <= fine cell side)
-//                         from the 9 contiguous neighbour rows of the fine
-//                         grid; for het beads the long-range het-het pairs
-//                         from the coarse grid that holds het beads only; LBS
-//                         binding, bonds/bending from the bead-ordered array,
-//                         the wall, the Heun update, and binning of the new
-//                         position into the next fine and het grids
-//                         (wave-aggregated atomics on per-cell counters and
-//                         per-scan-block totals, bucket slot).
-//   k_order               one thread per cell of both grids: exclusive scan
-//                         of the counts (block prefix from the per-block
-//                         totals), then a rank-by-bead-index ordering of each
-//                         cell's bucket, so the ordered arrays equal a stable
-//                         sort by cell (the result of ccp's
-//                         cub::DeviceRadixSort::SortPairs over identity values,
-//                         sugrid.cu:125-133) bit for bit, and a gather of the
-//                         positions into cell order.
-// Splitting the pair sum by cutoff keeps the result of ccp's single 27-cell
-// scan (calc_srf, chrsim.cu:277-317) while a short-range candidate costs a
-// 1.25-sigma cell instead of a 2-2.5-sigma one.
-// Reference behaviour followed: chrsim.cu:13-416, sugrid.cu:15-133.
+//   k_force<MODE, STAGE>  three lanes per bead, beads in grid order: lane p
+//                         scans the 3 neighbour rows of plane z + p - 1 of
+//                         the all-bead grid (ccp's 27-cell neighbourhood,
+//                         calc_srf, chrsim.cu:277-317) and the LBS sites
+//                         binned on the same geometry; lane 0 adds bonds,
+//                         bending and the wall; two shuffles sum the parts.
+//                         Then the Heun update, and binning of the new
+//                         position into the next grid (wave-aggregated
+//                         atomics on per-cell counters and per-scan-block
+//                         totals, bucket slot). Optionally (fine_cell_size <
+//                         cell_size) the pair sum is split by cutoff: short
+//                         pairs from a fine all-bead grid, the long het-het
+//                         tail from a coarse grid of het beads only.
+//   k_order               one thread per cell: exclusive scan of the counts
+//                         (block prefix from the per-block totals), then a
+//                         rank-by-bead-index ordering of each cell's bucket,
+//                         so the ordered arrays equal a stable sort by cell
+//                         (the result of ccp's cub::DeviceRadixSort::SortPairs
+//                         over identity values, sugrid.cu:125-133) bit for
+//                         bit, and a gather of the positions into cell order.
+// Workgroup indices are remapped so each XCD works on one contiguous run of
+// the grid (its L2 holds that run's neighbour rows).
+// Analysis kernels (chrana.cu): frame observables with FP64 sums, d(s)/P(s)
+// and the packed contact map in ccp's FP32 arithmetic.
+// Reference behaviour followed: chrsim.cu:13-416, sugrid.cu:15-133,
+// chrana.cu:13-84,372-512.
 
 #include "kernels.hpp"
 
@@ -139,7 +143,7 @@ __device__ __forceinline__ void bin_into(const GridGeom &g, uint32_t *count, uin
                                          uint32_t *flags, uint32_t i, float x, float y, float z) {
   bool outside = false;
   const int gx = cell_coord(x, g, outside), gy = cell_coord(y, g, outside), gz = cell_coord(z, g, outside);
-  const uint32_t key = (uint32_t)((gz * g.cps + gy) * g.cps + gx);
+  const uint32_t key = (uint32_t)(g.coff + (gz * g.cps + gy) * g.cps + gx);
   const uint32_t slot = agg_add(count, key);
   if (slot < (uint32_t)g.cap) bucket[(size_t)key * g.cap + slot] = i;
   agg_add_noret(blocksum, key / SCAN_BLOCK);
@@ -171,7 +175,7 @@ __device__ __forceinline__ void neighbour_rows(const GridGeom &g, const uint32_t
   for (int r = 0; r < 9; ++r) {
     const int zz = cz + r / 3 - 1, yy = cy + r % 3 - 1;
     const bool ok = zz >= 0 && zz < g.cps && yy >= 0 && yy < g.cps;
-    const int row = (zz * g.cps + yy) * g.cps;
+    const int row = g.coff + (zz * g.cps + yy) * g.cps;
     R.c0[r] = ok ? row + max(cx - 1, 0) : 0;
     R.c1[r] = ok ? row + min(cx + 1, g.cps - 1) + 1 : 0;
     R.lo[r] = cstart[R.c0[r]];
@@ -443,7 +447,7 @@ __device__ __forceinline__ void plane_rows(const GridGeom &g, const uint32_t *__
   for (int r = 0; r < 3; ++r) {
     const int yy = cy + r - 1;
     const bool ok = zz >= 0 && zz < g.cps && yy >= 0 && yy < g.cps;
-    const int row = (zz * g.cps + yy) * g.cps;
+    const int row = g.coff + (zz * g.cps + yy) * g.cps;
     R.c0[r] = ok ? row + max(cx - 1, 0) : 0;
     R.c1[r] = ok ? row + min(cx + 1, g.cps - 1) + 1 : 0;
     R.lo[r] = cstart[R.c0[r]];
@@ -488,62 +492,98 @@ __device__ __forceinline__ void scan_pass(const KParams &p, const float4 *__rest
   }
 }
 
-// Three lanes per bead: lane part p scans the neighbour rows of plane
-// z + p - 1; the partial forces are summed with two shuffles. 21 beads per
+// LANES = 3: three lanes per bead, lane part p scans the neighbour rows of
+// plane z + p - 1 and two shuffles sum the partial forces; 21 beads per
 // wavefront (lane 63 idles), so N beads give 3N/63 wavefronts to hide the
-// gather latency.
-constexpr int BEADS_PER_WAVE = 21;
-constexpr int BEADS_PER_BLOCK = BEADS_PER_WAVE * (BEAD_BLOCK / 64);
+// gather latency. LANES = 1: one lane scans all three planes.
+template <int LANES> struct Lanes {
+  static constexpr int BEADS_PER_WAVE = LANES == 3 ? 21 : 64;
+  static constexpr int BEADS_PER_BLOCK = BEADS_PER_WAVE * (BEAD_BLOCK / 64);
+};
+
+// System of cell-ordered slot t in a batch: the last s with sys_slot[s] <= t.
+__device__ __forceinline__ int system_of(const uint32_t *__restrict__ sys_slot, int n_sys, uint32_t t) {
+  int lo = 0, hi = n_sys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sys_slot[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 // Force evaluation of one stage, beads in grid order.
 // STAGE 0: write F(pos) by bead index. STAGE 1: Heun predictor
 // (exec_RK_1, chrsim.cu:364-388) er = r + f dt + rn, keeps
 // r + f dt/2 + rn. STAGE 2: corrector (exec_RK_2, :391-416)
 // r = (r + f dt/2 + rn) + ef dt/2. Stages 1 and 2 bin their output.
-template <int MODE, int STAGE, bool SPLIT>
+// BATCH: the slot's system supplies the nucleus geometry, eps_het, eps_lam,
+// the noise key and the grid (its cells are a range of the shared ones);
+// the noise counter uses the bead's index within its system, so a system
+// evolves exactly as it would alone with the same key.
+template <int MODE, int STAGE, bool SPLIT, bool BATCH, int LANES>
 __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
+  constexpr int BPW = Lanes<LANES>::BEADS_PER_WAVE;
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
-  const int part = lane % 3, slot = lane / 3;
-  const int t = (xcd_block() * (BEAD_BLOCK / 64) + wave) * BEADS_PER_WAVE + slot;
-  const bool active = slot < BEADS_PER_WAVE && t < p.N;
+  const int part = LANES == 3 ? lane % 3 : 0, slot = LANES == 3 ? lane / 3 : lane;
+  const int t = (xcd_block() * (BEAD_BLOCK / 64) + wave) * BPW + slot;
+  const bool active = slot < BPW && t < p.N;
   const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const uint32_t w = wbits(me);
   const uint32_t i = w & ID_MASK;
   const float rx = me.x, ry = me.y, rz = me.z;
+  uint32_t id0 = 0;
+  if constexpr (BATCH) {
+    const SysParams &S = q.sys[system_of(q.sys_slot, q.n_sys, active ? (uint32_t)t : 0u)];
+    p.R_n = S.R_n; p.R_o = S.R_o; p.R_b = S.R_b;
+    p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
+    p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
+    p.seed_lo = S.seed_lo; p.seed_hi = S.seed_hi;
+    p.gf = S.g;
+    id0 = S.slot;
+  }
 
-  Rows3 R;
-  plane_rows(p.gf, q.fcstart, rx, ry, rz, part - 1, R);
-  if (!active)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) R.hi[r] = R.lo[r];
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   if (active && part == 0) {
     bonded_force(p, q.pos, i, w, rx, ry, rz, fx, fy, fz);
     wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
   }
-  scan_pass<MODE, 0, SPLIT>(p, q.fspos, R, rx, ry, rz, w, fx, fy, fz);
-  if (MODE != MODE_ICG && active && p.n_l > 0 && (w & W_LAD)) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const uint32_t llo = q.lstart[R.c0[r]], lhi = q.lstart[R.c1[r]];
-      for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
+  for (int pl = 0; pl < (LANES == 3 ? 1 : 3); ++pl) {
+    const int dz = LANES == 3 ? part - 1 : pl - 1;
+    Rows3 R;
+    plane_rows(p.gf, q.fcstart, rx, ry, rz, dz, R);
+    if (!active)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) R.hi[r] = R.lo[r];
+    scan_pass<MODE, 0, SPLIT>(p, q.fspos, R, rx, ry, rz, w, fx, fy, fz);
+    if (!BATCH && MODE != MODE_ICG && active && p.n_l > 0 && (w & W_LAD)) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const uint32_t llo = q.lstart[R.c0[r]], lhi = q.lstart[R.c1[r]];
+        for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
+      }
+    }
+    if constexpr (SPLIT && MODE != MODE_ICG) {
+      if (active && (w & W_HET)) {
+        Rows3 H;
+        plane_rows(p.gh, q.hcstart, rx, ry, rz, dz, H);
+        scan_pass<MODE, 1, true>(p, q.hspos, H, rx, ry, rz, w, fx, fy, fz);
+      }
     }
   }
-  if constexpr (SPLIT && MODE != MODE_ICG) {
-    if (active && (w & W_HET)) {
-      Rows3 H;
-      plane_rows(p.gh, q.hcstart, rx, ry, rz, part - 1, H);
-      scan_pass<MODE, 1, true>(p, q.hspos, H, rx, ry, rz, w, fx, fy, fz);
-    }
+  if constexpr (LANES == 3) {
+    // sum the three parts into lane part 0
+    const int s1 = min(lane + 1, 63), s2 = min(lane + 2, 63);
+    const float ax = __shfl(fx, s1), ay = __shfl(fy, s1), az = __shfl(fz, s1);
+    const float bx = __shfl(fx, s2), by = __shfl(fy, s2), bz = __shfl(fz, s2);
+    if (!active || part != 0) return;
+    fx += ax + bx;
+    fy += ay + by;
+    fz += az + bz;
+  } else {
+    if (!active) return;
   }
-  // sum the three parts into lane part 0
-  const int s1 = min(lane + 1, 63), s2 = min(lane + 2, 63);
-  const float ax = __shfl(fx, s1), ay = __shfl(fy, s1), az = __shfl(fz, s1);
-  const float bx = __shfl(fx, s2), by = __shfl(fy, s2), bz = __shfl(fz, s2);
-  if (!active || part != 0) return;
-  fx += ax + bx;
-  fy += ay + by;
-  fz += az + bz;
   if (STAGE != 0 && p.fcap2 > 0.0f) {
     // optional safeguard of the WCA/LJ set: |F| <= fcap
     const float m2 = fx * fx + fy * fy + fz * fz;
@@ -559,7 +599,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
     if (p.sd > 0.0f) {
       float z0, z1, z2;
-      noise3(p.seed_lo, p.seed_hi, i, *q.step, z0, z1, z2);
+      noise3(p.seed_lo, p.seed_hi, i - id0, *q.step, z0, z1, z2);
       nx = p.sd * z0; ny = p.sd * z1; nz = p.sd * z2;
     }
     const float ex = rx + fx * p.dt + nx, ey = ry + fy * p.dt + ny, ez = rz + fz * p.dt + nz;
@@ -583,6 +623,18 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_bin(GridGeom g, int n_obj, const
   const float4 v = pos[i];
   if (het_only && !(wbits(v) & W_HET)) return;
   bin_into(g, count, blocksum, bucket, flags, (uint32_t)i, v.x, v.y, v.z);
+}
+
+// Binning of the index-ordered beads of a batch into their systems' grids
+// (bead i belongs to the system whose slot range holds i).
+__global__ __launch_bounds__(BEAD_BLOCK) void k_bin_batch(const SysParams *sys, const uint32_t *sys_slot, int n_sys,
+                                                          int n, const float4 *pos, uint32_t *count,
+                                                          uint32_t *blocksum, uint32_t *bucket, uint32_t *flags) {
+  const int i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = pos[i];
+  const SysParams &S = sys[system_of(sys_slot, n_sys, (uint32_t)i)];
+  bin_into(S.g, count, blocksum, bucket, flags, (uint32_t)i, v.x, v.y, v.z);
 }
 
 // Per-cell scan + deterministic order + gather of one or two grids
@@ -679,13 +731,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
 // rad = ceil(rc / csl) cells, FP32 without fused operations so the decision
 // matches the oracle bit for bit.
 __global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4 *spos, const uint32_t *cstart,
-                                                         uint32_t *map, int rad) {
+                                                         uint32_t *map, int rad, uint32_t slot0, uint32_t id0) {
 #pragma clang fp contract(off)
   const int t = xcd_block() * BEAD_BLOCK + threadIdx.x;
   if (t >= p.N) return;
   const GridGeom &g = p.gf;
-  const float4 me = spos[t];
-  const uint32_t i = wbits(me) & ID_MASK;
+  const float4 me = spos[slot0 + t];
+  const uint32_t i = (wbits(me) & ID_MASK) - id0;
   bool outside = false;
   const int cx = cell_coord(me.x, g, outside), cy = cell_coord(me.y, g, outside), cz = cell_coord(me.z, g, outside);
   const uint32_t nb = (uint32_t)p.contact_bins, bi = i / (uint32_t)p.contact_bin_size;
@@ -695,11 +747,11 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4
     for (int dy = -rad; dy <= rad; ++dy) {
       const int y = cy + dy;
       if (y < 0 || y >= g.cps) continue;
-      const int row = (z * g.cps + y) * g.cps;
+      const int row = g.coff + (z * g.cps + y) * g.cps;
       const uint32_t lo = cstart[row + max(cx - rad, 0)], hi = cstart[row + min(cx + rad, g.cps - 1) + 1];
       for (uint32_t k = lo; k < hi; ++k) {
         const float4 o = spos[k];
-        const uint32_t j = wbits(o) & ID_MASK;
+        const uint32_t j = (wbits(o) & ID_MASK) - id0;
         if (j <= i) continue;
         const float dx = me.x - o.x, dy2 = me.y - o.y, dz2 = me.z - o.z;
         const float xx = dx * dx, yy = dy2 * dy2, zz = dz2 * dz2;
@@ -905,15 +957,27 @@ void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t 
   k_order<<<blocks, SCAN_BLOCK, 0, s>>>(a, jb, flags, step_inc);
 }
 
+template <int MODE, bool SPLIT, bool BATCH, int LANES>
+static void force_variant(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
+  const unsigned g = blocks_for(p.N, Lanes<LANES>::BEADS_PER_BLOCK);
+  if (stage == 0) k_force<MODE, 0, SPLIT, BATCH, LANES><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else if (stage == 1) k_force<MODE, 1, SPLIT, BATCH, LANES><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else k_force<MODE, 2, SPLIT, BATCH, LANES><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+}
+
 template <int MODE, bool SPLIT> static void force_mode(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
-  const unsigned g = blocks_for(p.N, BEADS_PER_BLOCK);
-  if (stage == 0) k_force<MODE, 0, SPLIT><<<g, BEAD_BLOCK, 0, s>>>(p, q);
-  else if (stage == 1) k_force<MODE, 1, SPLIT><<<g, BEAD_BLOCK, 0, s>>>(p, q);
-  else k_force<MODE, 2, SPLIT><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  if (p.lanes == 1) force_variant<MODE, SPLIT, false, 1>(s, stage, p, q);
+  else force_variant<MODE, SPLIT, false, 3>(s, stage, p, q);
 }
 
 void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q) {
   if (p.N <= 0) return;
+  if (q.sys) {
+    // batches: the WCA/LJ set on one all-bead grid per system
+    if (p.lanes == 1) force_variant<MODE_LJ, false, true, 1>(s, stage, p, q);
+    else force_variant<MODE_LJ, false, true, 3>(s, stage, p, q);
+    return;
+  }
   if (mode == MODE_ICG) force_mode<MODE_ICG, false>(s, stage, p, q);
   else if (p.split) {
     if (mode == MODE_CCP) force_mode<MODE_CCP, true>(s, stage, p, q);
@@ -924,10 +988,11 @@ void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KP
   }
 }
 
-void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map) {
+void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map,
+                     uint32_t slot0, uint32_t id0) {
   if (p.N <= 0) return;
   const int rad = (int)ceilf(p.contact_rc / p.gf.csl);
-  k_contacts<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, map, rad);
+  k_contacts<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, map, rad, slot0, id0);
 }
 
 void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, float dmin2,
@@ -957,6 +1022,13 @@ void launch_sd_cp(hipStream_t s, const float4 *pos, uint32_t N, uint32_t lsdcp, 
 void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm, double *sum) {
   if (lcm == 0) return;
   k_ccp_cm<<<blocks_for((int)lcm, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(N, lcm, pos, cm, sum);
+}
+
+void launch_bin_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n,
+                      const float4 *pos, uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags) {
+  if (n <= 0) return;
+  k_bin_batch<<<blocks_for(n, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(sys, sys_slot, n_sys, n, pos, count, blocksum, bucket,
+                                                              flags);
 }
 
 void launch_close_sums(hipStream_t s, double *sum, double *m1, double *m2, uint32_t n, double n_dp) {

@@ -30,10 +30,23 @@ enum Mode : int { MODE_CCP = 0, MODE_ICG = 1, MODE_LJ = 2 };
 
 // Geometry of one uniform cell grid centred on the nucleus (ccp's sugrid,
 // sugrid.cuh:15-54): cell (gx, gy, gz) = floor(r / csl) + cps/2, index
-// gx + cps (gy + cps gz).
+// coff + gx + cps (gy + cps gz). coff places the grid of one system inside
+// the concatenated cell range of a batch (0 for a single system).
 struct GridGeom {
   float csl;
-  int cps, half, n_c, cap, nblk;
+  int cps, half, n_c, cap, nblk, coff;
+};
+
+// Per-system parameters of a batch of independent systems that share one
+// bead array and one concatenated cell range. Beads of system s hold the
+// slots [slot, slot + n) of the cell-ordered arrays and the bead indices
+// [slot, slot + n) of the index-ordered ones.
+struct SysParams {
+  float R_n, R_o, R_b, noc, boc, nod, bod;
+  float eps_het, eps_lam;
+  uint32_t seed_lo, seed_hi;
+  uint32_t slot, n;
+  GridGeom g;
 };
 
 // Device buffers of one grid. count/blocksum/bucket receive the binning of
@@ -52,6 +65,7 @@ struct KParams {
   GridGeom gf; // grid of every bead (all pairs, or the short-range pass when split), LBS sites
   GridGeom gh; // grid of het beads (long-range het-het pass when split)
   int split;   // 1: short-range pairs on gf + het-het tail on gh; 0: every pair on gf
+  int lanes;   // lanes per bead of the force kernel: 3 (one per z-plane) or 1
   // nucleus geometry (cngeom, chrdat.cuh:48-69)
   float R_n, R_o, R_b, noc, boc, nod, bod;
   // bonds and bending
@@ -87,6 +101,10 @@ struct KPtrs {
   uint32_t *hcount, *hblocksum, *hbucket; // ... and into the next het grid
   uint32_t *flags;         // FLAG_COUNT error counters
   const uint64_t *step;    // global step counter (noise counter)
+  // batches only (sys == nullptr for a single system)
+  const SysParams *sys;    // n_sys systems
+  const uint32_t *sys_slot; // n_sys + 1 first slots
+  int n_sys;
 };
 
 // One grid's ordering work inside a k_order launch.
@@ -107,7 +125,10 @@ void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, c
                 uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags);
 void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc);
 void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q);
-void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map);
+// Contacts of the beads in slots [slot0, slot0 + p.N) (bead indices from
+// id0) on p.gf, binned by (index - id0) / contact_bin_size.
+void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map,
+                     uint32_t slot0 = 0, uint32_t id0 = 0);
 void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, float dmin2,
                      unsigned long long *count);
 void launch_noise(hipStream_t s, const KParams &p, uint64_t step, float4 *out);
@@ -116,6 +137,8 @@ void launch_observables(hipStream_t s, int N, double R_n, double nod, double a, 
 void launch_sd_cp(hipStream_t s, const float4 *pos, uint32_t N, uint32_t lsdcp, uint32_t i_s, uint32_t i_e,
                   float *sd, float *cp, double *sum_sd, double *sum_cp);
 void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm, double *sum);
+void launch_bin_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n,
+                      const float4 *pos, uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags);
 void launch_close_sums(hipStream_t s, double *sum, double *m1, double *m2, uint32_t n, double n_dp);
 
 } // namespace ccpmi

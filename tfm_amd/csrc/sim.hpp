@@ -1,0 +1,263 @@
+// Internal definition of the simulation handle (struct ccp_sim) and the
+// host helpers of its initial-condition generator, shared by the C ABI
+// (ccpmi.cpp) and the batch engine (batch.cpp).
+#pragma once
+
+#include "host.hpp"
+#include "kernels.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+namespace ccpmi {
+namespace host {
+
+
+// ---------------------------------------------------------------- host Philox stream
+// Uniforms for the initial condition: Philox4x32-10 with the simulation seed
+// as key and counter {n lo, n hi, 0xffffffff, 1 << 16}, disjoint from the
+// step-noise counters {bead, step lo, step hi, attempt < 2^16}.
+struct HostRng {
+  uint32_t k0, k1;
+  uint64_t n = 0;
+  uint32_t buf[4];
+  int pos = 4;
+  explicit HostRng(uint64_t seed) : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)) {}
+  void refill() {
+    uint32_t c[4] = {(uint32_t)n, (uint32_t)(n >> 32), 0xffffffffu, 1u << 16};
+    uint32_t a = k0, b = k1;
+    for (int r = 0; r < 10; ++r) {
+      uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+      uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ a, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ b;
+      c[1] = (uint32_t)p1;
+      c[3] = (uint32_t)p0;
+      c[0] = n0;
+      c[2] = n2;
+      a += 0x9E3779B9u;
+      b += 0xBB67AE85u;
+    }
+    std::memcpy(buf, c, sizeof(c));
+    ++n;
+    pos = 0;
+  }
+  // uniform in (0, 1] like curandGenerateUniform
+  float uniform() {
+    if (pos == 4) refill();
+    return (float)((buf[pos++] >> 8) + 1u) * (1.0f / 16777216.0f);
+  }
+};
+
+// inverse error function: Winitzki's closed-form start (2008) refined by
+// Newton steps on std::erf
+inline double erfinv(double x) {
+  if (x <= -1.0) return -INFINITY;
+  if (x >= 1.0) return INFINITY;
+  const double a = 0.147, ln = std::log(1.0 - x * x), t = 2.0 / (M_PI * a) + ln / 2.0;
+  double y = std::copysign(std::sqrt(std::sqrt(t * t - ln / a) - t), x);
+  for (int k = 0; k < 4; ++k) y -= (std::erf(y) - x) / (2.0 / std::sqrt(M_PI) * std::exp(-y * y));
+  return y;
+}
+
+struct V3 {
+  float x, y, z;
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+
+} // namespace host
+} // namespace ccpmi
+
+// internal header: the library's own sources use these names unqualified
+using namespace ccpmi;
+using namespace ccpmi::host;
+
+
+// ---------------------------------------------------------------- handle
+struct ccp_sim {
+  // parameters
+  uint32_t N = 0, n_l = 0, fpf = 0, spf = 0, tpb = 0;
+  float T = 0, R_n = 0, R_o = 0, R_b = 0;
+  float noc = 0, boc = 0, nod = 0, bod = 0, d_m = 0;
+  float dt = 0, sd = 0;
+  uint64_t seed = 0;
+  int ff = CCP_FF_CCP;
+  float csl = 0;   // coarse cell side: het grid and ccp-layout export grid
+  float csl_f = 0; // cell side of the all-bead grid of the step (fine when split)
+  bool split = false; // short-range pairs on the fine grid + het-het tail on the het grid
+  std::vector<uint32_t> chain_off;
+  std::string seq_file;
+  int seq_mode = 0; // 0 ccp, 1 het/LAD generator
+  float het_fraction = 0.3f, het_mean = 20.0f, lad_fraction = 0.1f;
+  int lad_ends = 1;
+  float k_b = K_BEND;
+  int bend_all = 0, bond_type = 0;
+  float eps_het = 0.5f, eps_lam = 1.0f;
+  float contact_rc = 0;
+  uint32_t cbins = 0, cbin_size = 0, cinterval = 0;
+  uint32_t graph_steps = 64;
+  uint32_t lanes = 3; // lanes per bead of the force kernel
+  float fcap = 0.0f;
+  KParams kp{};
+
+  // state
+  uint32_t i_f = 0;
+  float t = 0.0f;
+  uint64_t step = 0;
+  std::vector<float> hr, hlr;
+  std::vector<uint8_t> hfl;
+  bool lgrid_valid = false, dstep_dirty = true;
+  uint32_t n_samples = 0;
+
+  // device
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  DevBuf<float4> r, er, pstar, lr, lspos, aux;
+  DevBuf<uint32_t> lstart, flags, cmap;
+  DevBuf<uint64_t> dstep;
+  DevBuf<unsigned long long> dcount;
+  // one uniform grid: binning buffers + ordered cell list
+  struct HostGrid {
+    GridGeom g{};
+    DevBuf<uint32_t> count, bs0, bs1, bucket, cstart, soi;
+    DevBuf<float4> spos;
+    int clean = 0;       // which per-block total buffer is zero
+    bool valid = false;  // holds the cell list of the current positions
+    bool allocated = false;
+    uint32_t *bs(int k) { return k ? bs1.p : bs0.p; }
+    void alloc(size_t n_obj) {
+      count.alloc(g.n_c); bs0.alloc(g.nblk); bs1.alloc(g.nblk);
+      bucket.alloc((size_t)g.n_c * g.cap); cstart.alloc(g.n_c + 1); soi.alloc(n_obj); spos.alloc(n_obj);
+      hip_check(hipMemset(count.p, 0, (size_t)g.n_c * 4));
+      hip_check(hipMemset(bs0.p, 0, (size_t)g.nblk * 4));
+      hip_check(hipMemset(bs1.p, 0, (size_t)g.nblk * 4));
+      hip_check(hipMemset(cstart.p, 0, ((size_t)g.n_c + 1) * 4));
+      clean = 0;
+      valid = false;
+      allocated = true;
+    }
+    void free_() {
+      DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &bucket, &cstart, &soi};
+      for (auto *b : u) b->free_();
+      spos.free_();
+      allocated = false;
+    }
+  };
+  HostGrid gf; // fine grid of every bead
+  GridGeom lg{}; // LBS sites on the fine geometry (uses gf's binning buffers)
+  HostGrid gh; // coarse grid of het beads
+  HostGrid gx; // coarse grid of every bead in ccp's geometry (cell-list export)
+  hipGraphExec_t graphs[3][2][2] = {};
+
+  // misc
+  std::string err;
+  ccp_log_fn log_fn = nullptr;
+  void *log_user = nullptr;
+
+  void drop_graphs() {
+    for (auto &gm : graphs)
+      for (auto &gr : gm)
+        for (auto &g : gr)
+          if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+          }
+  }
+
+  ~ccp_sim() {
+    drop_graphs();
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &lr, &lspos, &aux};
+    for (auto *b : f4) b->free_();
+    DevBuf<uint32_t> *u4[] = {&lstart, &flags, &cmap};
+    for (auto *b : u4) b->free_();
+    gf.free_();
+    gh.free_();
+    gx.free_();
+    dstep.free_();
+    dcount.free_();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // -------------------------------------------------------------- logging
+  void record(const std::string &msg) {
+    if (log_fn) {
+      log_fn(msg.c_str(), log_user);
+      return;
+    }
+    time_t now = time(nullptr);
+    char ts[21];
+    strftime(ts, 21, "[%d/%m/%y %H:%M:%S] ", localtime(&now));
+    std::cout << ts << msg << std::endl;
+  }
+  void progress(float pc) {
+    std::string line = "progress: " + cnfs(pc, 5, '0', 1) + "%";
+    if (log_fn) {
+      log_fn(line.c_str(), log_user);
+      return;
+    }
+    std::cout << line << "\r";
+    std::cout.flush();
+  }
+
+  // -------------------------------------------------------------- setup
+  void configure(const ParMap &par);
+  void allocate();
+  void fill_kparams();
+  uint32_t wbits(uint32_t i) const;
+  void upload_positions();
+  void upload_lbs();
+  void download_positions();
+  void check_flags();
+  // -------------------------------------------------------------- step machinery
+  KParams mode_params(int mode) const {
+    KParams k = kp;
+    if (mode == MODE_ICG) k.sd = 0.0f;
+    return k;
+  }
+  void build_grid(HostGrid &G, const GridGeom &geom, const float4 *pos, int n_obj, bool het_only, float4 *out_spos,
+                  uint32_t *out_cstart, uint32_t *out_soi);
+  bool take_overflow();
+  void grow_capacity();
+  void ensure_grids();
+  void invalidate() { gf.valid = gh.valid = gx.valid = false; }
+  void sync_step();
+  void launch_stage(int mode, int stage);
+  void launch_step(int mode) {
+    launch_stage(mode, 1);
+    launch_stage(mode, 2);
+  }
+  void run_steps(uint64_t n, int mode);
+  void run_chunk(uint64_t n, int mode);
+  // -------------------------------------------------------------- initial condition
+  void set_lbs_positions(HostRng &g);
+  void set_particle_types(HostRng &g);
+  void set_particle_positions(HostRng &g);
+  void perform_random_walk(HostRng &g, uint32_t i_s, uint32_t i_e, V3 dir, float mda);
+  uint64_t count_overlaps(float dmin);
+  void generate_initial_condition();
+  // -------------------------------------------------------------- files
+  void write_frame_bin(std::ofstream &f);
+  // -------------------------------------------------------------- analysis
+  // bead range of ccp's chromosome c for d(s)/P(s) (chrana.cu:93-97,484-491)
+  bool sd_cp_range(uint32_t c, uint32_t &i_s, uint32_t &i_e) const {
+    if (N == N_DEF) {
+      if (c >= 6) return false;
+      i_s = CHR_LIM[c];
+      i_e = CHR_LIM[c + 1];
+      return true;
+    }
+    if (c != 0) return false;
+    i_s = 0;
+    i_e = N;
+    return true;
+  }
+};
