@@ -6,9 +6,13 @@ Workload (the large-nucleus configuration): 8 chains of 16384 beads
 ccp's formula), WCA repulsion, heterochromatin LJ attraction eps = 0.6 kT at
 2.5 sigma (30 % het in Geometric(20) blocks), 15 % LAD beads clustered at
 chain ends with a lamina well, harmonic bonds, bending k_b = 2 kT, dt = 1e-3,
-2.5 sigma cells, FP32, contact sampling every 1000 steps into a 1024 x 1024
-uint32 map (128 beads per bin), seed 3. One step = one stochastic-Heun
-step of all beads (two force evaluations, two cell-list builds).
+neighbour search over 2.5 sigma cells (the autotuner may split each into
+2 x 2 x 2 sub-cells), FP32, contact sampling every 1000 steps into a
+1024 x 1024 uint32 map (128 beads per bin), seed 3. One step = one
+stochastic-Heun step of all beads (two force evaluations, two cell-list
+builds). The force kernel's shape (cell side, reach, lanes per bead) is
+picked by tfm_amd.autotune on the workload's initial condition before the
+timed run; --no-tune keeps the library default.
 
 Prints one JSON line. `value` is bead-steps/s with the state resident in HBM;
 `roofline` is the dominant kernel's algorithmic bytes per launch over its
@@ -45,7 +49,6 @@ PARAMS = {
     "lad_fraction": 0.15,
     "lad_placement": "ends",
     "timestep": 0.001,
-    "cell_size": 2.5,
     "contact_cutoff": 1.5,
     "contact_bin_size": 128,
     "contact_interval": 1000,
@@ -77,6 +80,7 @@ def parse():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_force_lj_n131072_traffic.txt"))
     ap.add_argument("--param", action="append", default=[], metavar="KEY=VALUE",
                     help="override a simulation parameter (e.g. fine_cell_size=1.25)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the kernel-shape autotuner")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain elementwise kernel of known traffic (256 MiB read + 256 MiB "
                          "written per launch, 10 launches) to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -119,7 +123,7 @@ def cpu_baseline(sim, budget_s):
     cores = min(len(os.sched_getaffinity(0)), 16)
     r = sim.positions.astype(np.float64)
     osys = po.System(N, float(info.R_n), flags=sim.flags, chain_off=sim.chains.astype(np.int32), ff=po.FF_LJ,
-                     csl=float(info.csl), seed=int(info.seed), dt=float(info.dt), bend_all=1, k_b=2.0,
+                     csl=2.5, seed=int(info.seed), dt=float(info.dt), bend_all=1, k_b=2.0,
                      eps_het=0.6, eps_lam=1.0, fcap=float(info.force_cap))
     step0 = sim.time[2]
     t0 = time.perf_counter()
@@ -153,6 +157,19 @@ def main():
         params[k] = v
     sim = tfm_amd.Simulation(params)
     sim.generate_initial_condition()
+    shape, table = {}, []
+    if not args.no_tune and not any(k.split("=")[0] in ("cell_size", "cell_reach", "lanes_per_bead")
+                                    for k in args.param):
+        from tfm_amd import autotune
+
+        r0, f0 = sim.positions, sim.flags
+        shape, table = autotune.tune(params, r0, f0, cutoff=2.5,
+                                     log=lambda m: print(m, file=sys.stderr, flush=True))
+        sim.close()
+        sim = tfm_amd.Simulation({**params, **shape})
+        sim.flags = f0
+        sim.positions = r0
+        sim.set_time(0, 0.0, 0)
     print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
     sim.run_steps(args.warmup)
     sim.synchronize()
@@ -171,7 +188,8 @@ def main():
     # per-kernel device time (HIP events on the simulation's stream)
     kt = sim.time_kernels(200)
     force_ms = max(kt[0], kt[2])
-    dominant = "k_force<LJ,1>" if kt[0] >= kt[2] else "k_force<LJ,2>"
+    kname = "k_force_bal" if shape.get("lanes_per_bead") == 8 else "k_force"
+    dominant = f"{kname}<LJ,1>" if kt[0] >= kt[2] else f"{kname}<LJ,2>"
     bytes_force = BYTES_FORCE_PER_BEAD * N
     achieved = bytes_force / (force_ms * 1e-3) / 1e9
     traffic = read_traffic(args.traffic)
@@ -194,8 +212,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "ccp BD step, 8 chains x 16384 beads (N=131072), volume fraction 0.3, WCA + het LJ "
-                        "eps 0.6 at 2.5 sigma, LAD 15% at chain ends, dt 1e-3, 2.5 sigma cells, contacts "
-                        "every 1000 steps into 1024x1024 uint32",
+                        "eps 0.6 at 2.5 sigma, LAD 15% at chain ends, dt 1e-3, 2.5 sigma cell neighbourhood, "
+                        "contacts every 1000 steps into 1024x1024 uint32",
+            "kernel_shape": {"cell_size": float(sim.info.fine_csl), **{k: v for k, v in shape.items()
+                                                                        if k != "cell_size"}},
+            "autotune_us_per_step": [[c, round(t * 1e3, 2)] for c, t in table],
             "N": N, "chains": N_CHAINS, "R_n": round(R_N, 4), "cells_per_side": sim.info.cps,
             "contact_samples": n_samples, "contact_total": int(cmap.sum(dtype=np.uint64)),
             "overrides": args.param,
@@ -203,6 +224,8 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            # whole step at 176 algorithmic bytes per bead-step (index-array cell list)
+            "step_frac_176B": 176.0 * value / (HBM_PEAK_GBS * 1e9),
             "kernel_ms": {"force_stage1": float(kt[0]), "order_stage1": float(kt[1]),
                           "force_stage2": float(kt[2]), "order_stage2": float(kt[3]), "step": float(kt[4])},
             "bytes_per_launch": {"force": bytes_force, "order": order_bytes},

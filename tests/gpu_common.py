@@ -24,7 +24,7 @@ def make_pair(N, R_n, flags, r, ff=po.FF_CCP, lbs=None, chain_off=None, seed=9, 
         okw.update(bend_all=1, k_b=2.0, eps_het=kw.get("eps_het", 0.5), eps_lam=kw.get("eps_lam", 1.0))
     if dt is not None:
         params["timestep"] = repr(float(dt))
-    for k in ("opening_radius", "bleb_radius", "lanes_per_bead", "fine_cell_size"):
+    for k in ("opening_radius", "bleb_radius", "lanes_per_bead", "fine_cell_size", "cell_reach"):
         if k in kw:
             params[k] = kw[k]
     if chain_off is not None:
@@ -40,6 +40,7 @@ def make_pair(N, R_n, flags, r, ff=po.FF_CCP, lbs=None, chain_off=None, seed=9, 
                      chain_off=chain_off, seed=seed, dt=float(np.float32(dt if dt else 1 / 2048)),
                      R_o=kw.get("opening_radius", 0.0), R_b=kw.get("bleb_radius", 0.0),
                      fcap=float(sim.info.force_cap), **okw)
-    assert osys.p.cps == sim.info.cps or sim.info.cps == 4
+    # same grid as the oracle's cell list unless the test picks another cell shape
+    assert osys.p.cps == sim.info.cps or sim.info.cps == 4 or "cell_reach" in kw
     assert abs(osys.p.sd - sim.info.sd) < 1e-9
     return sim, osys, r32.astype(np.float64)

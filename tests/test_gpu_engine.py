@@ -70,9 +70,9 @@ def test_initial_condition_default_sequence():
     sim = tfm_amd.Simulation({"nucleus_radius": f"{R_n:.6f}", "sequence_file": seq, "seed": 4,
                               "number_of_lbs": laf_sites(R_n, 0.2)})
     assert list(sim.chains) == CHR_LIM
+    sim.generate_initial_condition()  # sets the types (set_particle_types, chrsim.cu:634-675)
     expect = np.array([0 if c == "A" else 1 for c in open(seq).read().strip()], np.int32)
     assert np.array_equal(sim.types, expect)
-    sim.generate_initial_condition()
     assert sim.count_overlaps(MIS) == 0
     r = sim.positions.astype(np.float64)
     assert np.linalg.norm(r, axis=1).max() < R_n
@@ -85,6 +85,7 @@ def test_het_lad_generator():
     R_n = radius_for(N, 0.3)
     sim = tfm_amd.Simulation({"number_of_particles": N, "nucleus_radius": f"{R_n:.6f}", "number_of_chains": 8,
                               "force_field": "lj", "lad_fraction": 0.15, "het_fraction": 0.3, "seed": 3})
+    sim.generate_initial_condition()
     fl = sim.flags
     het, lad = (fl & 1) > 0, (fl & 2) > 0
     assert abs(het.mean() - 0.3) < 0.05

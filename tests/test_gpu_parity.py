@@ -49,6 +49,18 @@ FORCE_CASES = [
     # one lane per bead (the autotuner's other kernel shape)
     dict(ff=po.FF_CCP, n_l=40, lanes_per_bead=1),
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, lanes_per_bead=1),
+    # 5 x 5 x 5 cells of half the side, a lane per plane or one lane per bead
+    dict(ff=po.FF_CCP, n_l=40, cell_reach=2),
+    dict(ff=po.FF_ICG, n_l=0, cell_reach=2),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=1),
+    # eight lanes per bead over a compacted list of trimmed rows
+    dict(ff=po.FF_CCP, n_l=40, lanes_per_bead=8),
+    dict(ff=po.FF_ICG, n_l=0, lanes_per_bead=8),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, lanes_per_bead=8),
+    dict(ff=po.FF_CCP, n_l=40, cell_reach=2, lanes_per_bead=8),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8),
+    dict(ff=po.FF_CCP, n_l=20, opening_radius=4.0, bleb_radius=6.0, lanes_per_bead=8),
     # pair sum split by cutoff over a fine grid and a het grid
     dict(ff=po.FF_CCP, n_l=40, fine_cell_size=1.2),
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, fine_cell_size=1.25),
@@ -85,9 +97,11 @@ def test_forces_match_oracle(case):
     sim.close()
 
 
-@pytest.mark.parametrize("ff,dt,tol", [(po.FF_CCP, 1 / 2048, 1e-3), (po.FF_LJ, 1e-3, 2e-3)])
-def test_trajectory_1000_steps(ff, dt, tol):
-    step(f"trajectory ff={ff}")
+@pytest.mark.parametrize("ff,dt,tol,reach,lanes", [(po.FF_CCP, 1 / 2048, 1e-3, 1, 3), (po.FF_LJ, 1e-3, 2e-3, 1, 3),
+                                                   (po.FF_LJ, 1e-3, 2e-3, 2, 5), (po.FF_CCP, 1 / 2048, 1e-3, 1, 8),
+                                                   (po.FF_LJ, 1e-3, 2e-3, 2, 8)])
+def test_trajectory_1000_steps(ff, dt, tol, reach, lanes):
+    step(f"trajectory ff={ff} reach={reach} lanes={lanes}")
     N = 1024
     R_n = radius_for(N, 0.2)
     r = relaxed(N, R_n, 2)
@@ -98,7 +112,7 @@ def test_trajectory_1000_steps(ff, dt, tol):
     else:
         fl = pattern(N, 1)
         lbs = lbs_sites(R_n, 30, 8)
-    sim, osys, r64 = make_pair(N, R_n, fl, r, ff=ff, dt=dt, lbs=lbs)
+    sim, osys, r64 = make_pair(N, R_n, fl, r, ff=ff, dt=dt, lbs=lbs, cell_reach=reach, lanes_per_bead=lanes)
     sim.set_time(0, 0.0, 0)
     sim.run_steps(1000)
     sim.synchronize()

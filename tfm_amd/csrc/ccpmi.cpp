@@ -89,17 +89,24 @@ void ccp_sim::configure(const ParMap &par) {
   contact_rc = par.get<float>("contact_cutoff", lj ? 1.5f : ACO);
   float cutoff = lj ? rc_lj : ACO;
   cutoff = std::max(std::max(cutoff, LCO), contact_rc);
-  csl = par.get<float>("cell_size", lj ? rc_lj : ACO);
-  if (!(csl >= cutoff)) throw Error(CCP_ERR_PARAM, "cell_size below the interaction cutoff");
+  // cell_reach 2 scans 5 x 5 x 5 cells of at least half the cutoff instead
+  // of ccp's 3 x 3 x 3 (the same pairs, fewer distant candidates)
+  reach = par.get<uint32_t>("cell_reach", 1u);
+  if (reach != 1 && reach != 2) throw Error(CCP_ERR_PARAM, "cell_reach out of range");
+  csl = par.get<float>("cell_size", (lj ? rc_lj : ACO) / reach);
+  if (!(csl * reach >= cutoff)) throw Error(CCP_ERR_PARAM, "cell_size below the interaction cutoff");
   // fine grid: side >= every short-range cutoff (pair repulsion, LBS
   // binding, the initial-overlap distance)
   // (the split is off when fine_cell_size equals cell_size, the default)
   const float short_cut = lj ? (float)std::pow(2.0, 1.0 / 6.0) : RCO;
   csl_f = par.get<float>("fine_cell_size", csl);
-  if (!(csl_f >= short_cut && csl_f >= LCO && csl_f >= MIS && csl_f >= contact_rc / 2.0f))
-    throw Error(CCP_ERR_PARAM, "fine_cell_size below the short-range cutoff");
   if (csl_f > csl) csl_f = csl;
   split = csl_f < csl;
+  if (split && !(csl_f >= short_cut && csl_f >= LCO && csl_f >= MIS && csl_f >= contact_rc / 2.0f))
+    throw Error(CCP_ERR_PARAM, "fine_cell_size below the short-range cutoff");
+  // the overlap count of the initial condition scans one cell around each bead
+  if (!(csl_f >= MIS)) throw Error(CCP_ERR_PARAM, "cell_size below the overlap distance");
+  if (split && reach != 1) throw Error(CCP_ERR_PARAM, "fine_cell_size needs cell_reach 1");
   uint32_t cap = par.get<uint32_t>("cell_capacity", 32u);
   uint32_t cap_f = par.get<uint32_t>("fine_cell_capacity", 16u);
   if (!(1 <= cap && cap <= 1024)) throw Error(CCP_ERR_PARAM, "cell_capacity out of range");
@@ -110,6 +117,7 @@ void ccp_sim::configure(const ParMap &par) {
     if (cps < 4) cps = 4;
     if ((uint64_t)cps * cps * cps >= (1ull << 31)) throw Error(CCP_ERR_PARAM, "cell_size too small for the nucleus");
     g.csl = side;
+    g.icsl = 1.0f / side;
     g.cps = (int)cps;
     g.half = (int)(cps / 2);
     g.n_c = (int)(cps * cps * cps);
@@ -119,12 +127,13 @@ void ccp_sim::configure(const ParMap &par) {
   };
   gh.g = geometry(csl, cap);
   gx.g = gh.g;
+  gx.g.icsl = 0.0f; // the exported cell list divides by csl as ccp does
   gf.g = geometry(csl_f, split ? cap_f : cap);
   lg = gf.g;
   fcap = par.get<float>("force_cap", lj ? 0.2f / dt : 0.0f);
   if (!(fcap >= 0.0f)) throw Error(CCP_ERR_PARAM, "force_cap out of range");
-  lanes = par.get<uint32_t>("lanes_per_bead", 3u);
-  if (lanes != 1 && lanes != 3) throw Error(CCP_ERR_PARAM, "lanes_per_bead out of range");
+  lanes = par.get<uint32_t>("lanes_per_bead", 2 * reach + 1);
+  if (lanes != 1 && lanes != 2 * reach + 1 && lanes != 8) throw Error(CCP_ERR_PARAM, "lanes_per_bead out of range");
   graph_steps = par.get<uint32_t>("graph_steps", 64u);
   if (!(1 <= graph_steps && graph_steps <= 4096)) throw Error(CCP_ERR_PARAM, "graph_steps out of range");
 
@@ -188,6 +197,7 @@ void ccp_sim::fill_kparams() {
   if (!split) k.gh.n_c = 0;
   k.split = split ? 1 : 0;
   k.lanes = (int)lanes;
+  k.reach = split ? 1 : (int)reach;
   k.R_n = R_n; k.R_o = R_o; k.R_b = R_b;
   k.noc = noc; k.boc = boc; k.nod = nod; k.bod = bod;
   const bool lj = ff == CCP_FF_LJ;

@@ -132,8 +132,10 @@ __device__ __forceinline__ void agg_add_noret(uint32_t *base, uint32_t key) {
 // ---------------------------------------------------------------- grid
 // Cell coordinate along one axis: floor(x / csl) + cps/2 as ccp's
 // calc_indexes (sugrid.cu:26-34), clamped into the grid (ccp has no clamp).
+// Step grids multiply by 1/csl instead of dividing: every lookup of a grid
+// uses the same rule, so binning and neighbour scans stay consistent.
 __device__ __forceinline__ int cell_coord(float x, const GridGeom &g, bool &outside) {
-  int c = (int)floorf(x / g.csl) + g.half;
+  int c = (int)floorf(g.icsl != 0.0f ? x * g.icsl : x / g.csl) + g.half;
   if (c < 0) { c = 0; outside = true; }
   if (c >= g.cps) { c = g.cps - 1; outside = true; }
   return c;
@@ -144,7 +146,8 @@ __device__ __forceinline__ void bin_into(const GridGeom &g, uint32_t *count, uin
   bool outside = false;
   const int gx = cell_coord(x, g, outside), gy = cell_coord(y, g, outside), gz = cell_coord(z, g, outside);
   const uint32_t key = (uint32_t)(g.coff + (gz * g.cps + gy) * g.cps + gx);
-  const uint32_t slot = agg_add(count, key);
+  // beads of one wavefront mostly sit in distinct cells: plain atomics
+  const uint32_t slot = atomicAdd(count + key, 1u);
   if (slot < (uint32_t)g.cap) bucket[(size_t)key * g.cap + slot] = i;
   agg_add_noret(blocksum, key / SCAN_BLOCK);
   if (outside) atomicAdd(flags + FLAG_OUTSIDE, 1u);
@@ -431,40 +434,43 @@ __device__ __forceinline__ int xcd_block() {
   return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + k;
 }
 
-// The 3 neighbour rows (y-1..y+1) of plane z + dz around the cell holding
-// (x, y, z); rows outside the grid get an empty range.
-struct Rows3 {
-  uint32_t lo[3], hi[3];
-  int c0[3], c1[3];
+// The 2R+1 neighbour rows (y-R..y+R, x-R..x+R) of plane z + dz around the
+// cell holding (x, y, z); rows outside the grid get an empty range. Reach R
+// = 1 covers ccp's 27 cells; R = 2 covers 125 cells of half the side (the
+// same cutoff sphere with fewer candidates).
+template <int R> struct RowsR {
+  uint32_t lo[2 * R + 1], hi[2 * R + 1];
+  int c0[2 * R + 1], c1[2 * R + 1];
 };
 
+template <int R>
 __device__ __forceinline__ void plane_rows(const GridGeom &g, const uint32_t *__restrict__ cstart, float x, float y,
-                                           float z, int dz, Rows3 &R) {
+                                           float z, int dz, RowsR<R> &Q) {
   bool outside = false;
   const int cx = cell_coord(x, g, outside), cy = cell_coord(y, g, outside), cz = cell_coord(z, g, outside);
   const int zz = cz + dz;
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const int yy = cy + r - 1;
+  for (int r = 0; r < 2 * R + 1; ++r) {
+    const int yy = cy + r - R;
     const bool ok = zz >= 0 && zz < g.cps && yy >= 0 && yy < g.cps;
     const int row = g.coff + (zz * g.cps + yy) * g.cps;
-    R.c0[r] = ok ? row + max(cx - 1, 0) : 0;
-    R.c1[r] = ok ? row + min(cx + 1, g.cps - 1) + 1 : 0;
-    R.lo[r] = cstart[R.c0[r]];
-    R.hi[r] = cstart[R.c1[r]];
+    Q.c0[r] = ok ? row + max(cx - R, 0) : 0;
+    Q.c1[r] = ok ? row + min(cx + R, g.cps - 1) + 1 : 0;
+    Q.lo[r] = cstart[Q.c0[r]];
+    Q.hi[r] = cstart[Q.c1[r]];
   }
 }
 
-// Candidates of 3 rows, four at a time (independent loads in flight).
+// Candidates of the rows, four at a time (independent loads in flight).
 // PASS 0: every pair on the all-bead grid (full cutoffs, or the short
 // cutoff when split); PASS 1: het-het tail on the het grid.
-template <int MODE, int PASS, bool SPLIT>
-__device__ __forceinline__ void scan_pass(const KParams &p, const float4 *__restrict__ spos, const Rows3 &R, float rx,
-                                          float ry, float rz, uint32_t w, float &fx, float &fy, float &fz) {
+template <int MODE, int PASS, bool SPLIT, int R>
+__device__ __forceinline__ void scan_pass(const KParams &p, const float4 *__restrict__ spos, const RowsR<R> &Q,
+                                          float rx, float ry, float rz, uint32_t w, float &fx, float &fy, float &fz) {
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    uint32_t k = R.lo[r];
-    const uint32_t h = R.hi[r];
+  for (int r = 0; r < 2 * R + 1; ++r) {
+    uint32_t k = Q.lo[r];
+    const uint32_t h = Q.hi[r];
     for (; k + 4u <= h; k += 4u) {
       const float4 a = spos[k], b = spos[k + 1], c = spos[k + 2], d = spos[k + 3];
       if constexpr (PASS == 1) {
@@ -492,12 +498,13 @@ __device__ __forceinline__ void scan_pass(const KParams &p, const float4 *__rest
   }
 }
 
-// LANES = 3: three lanes per bead, lane part p scans the neighbour rows of
-// plane z + p - 1 and two shuffles sum the partial forces; 21 beads per
-// wavefront (lane 63 idles), so N beads give 3N/63 wavefronts to hide the
-// gather latency. LANES = 1: one lane scans all three planes.
+// LANES = 2R+1: one lane per z-plane of the neighbourhood, the partial
+// forces summed with shuffles (R = 1: 21 beads per wavefront, lane 63
+// idles; R = 2: 12 beads, lanes 60-63 idle), so N beads give LANES N / 63
+// wavefronts to hide the gather latency. LANES = 1: one lane scans every
+// plane.
 template <int LANES> struct Lanes {
-  static constexpr int BEADS_PER_WAVE = LANES == 3 ? 21 : 64;
+  static constexpr int BEADS_PER_WAVE = 64 / LANES;
   static constexpr int BEADS_PER_BLOCK = BEADS_PER_WAVE * (BEAD_BLOCK / 64);
 };
 
@@ -521,11 +528,12 @@ __device__ __forceinline__ int system_of(const uint32_t *__restrict__ sys_slot, 
 // the noise key and the grid (its cells are a range of the shared ones);
 // the noise counter uses the bead's index within its system, so a system
 // evolves exactly as it would alone with the same key.
-template <int MODE, int STAGE, bool SPLIT, bool BATCH, int LANES>
+template <int MODE, int STAGE, bool SPLIT, bool BATCH, int R, int LANES>
 __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
+  static_assert(LANES == 1 || LANES == 2 * R + 1, "one lane per bead or one per plane");
   constexpr int BPW = Lanes<LANES>::BEADS_PER_WAVE;
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
-  const int part = LANES == 3 ? lane % 3 : 0, slot = LANES == 3 ? lane / 3 : lane;
+  const int part = LANES > 1 ? lane % LANES : 0, slot = LANES > 1 ? lane / LANES : lane;
   const int t = (xcd_block() * (BEAD_BLOCK / 64) + wave) * BPW + slot;
   const bool active = slot < BPW && t < p.N;
   const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -549,38 +557,44 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
     wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
   }
 #pragma unroll
-  for (int pl = 0; pl < (LANES == 3 ? 1 : 3); ++pl) {
-    const int dz = LANES == 3 ? part - 1 : pl - 1;
-    Rows3 R;
-    plane_rows(p.gf, q.fcstart, rx, ry, rz, dz, R);
+  for (int pl = 0; pl < (LANES > 1 ? 1 : 2 * R + 1); ++pl) {
+    const int dz = (LANES > 1 ? part : pl) - R;
+    RowsR<R> Q;
+    plane_rows<R>(p.gf, q.fcstart, rx, ry, rz, dz, Q);
     if (!active)
 #pragma unroll
-      for (int r = 0; r < 3; ++r) R.hi[r] = R.lo[r];
-    scan_pass<MODE, 0, SPLIT>(p, q.fspos, R, rx, ry, rz, w, fx, fy, fz);
+      for (int r = 0; r < 2 * R + 1; ++r) Q.hi[r] = Q.lo[r];
+    scan_pass<MODE, 0, SPLIT, R>(p, q.fspos, Q, rx, ry, rz, w, fx, fy, fz);
     if (!BATCH && MODE != MODE_ICG && active && p.n_l > 0 && (w & W_LAD)) {
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        const uint32_t llo = q.lstart[R.c0[r]], lhi = q.lstart[R.c1[r]];
+      for (int r = 0; r < 2 * R + 1; ++r) {
+        const uint32_t llo = q.lstart[Q.c0[r]], lhi = q.lstart[Q.c1[r]];
         for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
       }
     }
     if constexpr (SPLIT && MODE != MODE_ICG) {
       if (active && (w & W_HET)) {
-        Rows3 H;
-        plane_rows(p.gh, q.hcstart, rx, ry, rz, dz, H);
-        scan_pass<MODE, 1, true>(p, q.hspos, H, rx, ry, rz, w, fx, fy, fz);
+        RowsR<1> H;
+        plane_rows<1>(p.gh, q.hcstart, rx, ry, rz, dz, H);
+        scan_pass<MODE, 1, true, 1>(p, q.hspos, H, rx, ry, rz, w, fx, fy, fz);
       }
     }
   }
-  if constexpr (LANES == 3) {
-    // sum the three parts into lane part 0
-    const int s1 = min(lane + 1, 63), s2 = min(lane + 2, 63);
-    const float ax = __shfl(fx, s1), ay = __shfl(fy, s1), az = __shfl(fz, s1);
-    const float bx = __shfl(fx, s2), by = __shfl(fy, s2), bz = __shfl(fz, s2);
+  if constexpr (LANES > 1) {
+    // sum the plane parts into lane part 0 (lane + k stays in the wave; the
+    // idle top lanes contribute zero)
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+    for (int k = 1; k < LANES; ++k) {
+      const int src = min(lane + k, 63);
+      ax += __shfl(fx, src);
+      ay += __shfl(fy, src);
+      az += __shfl(fz, src);
+    }
     if (!active || part != 0) return;
-    fx += ax + bx;
-    fy += ay + by;
-    fz += az + bz;
+    fx += ax;
+    fy += ay;
+    fz += az;
   } else {
     if (!active) return;
   }
@@ -593,6 +607,172 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
     }
   }
 
+  if constexpr (STAGE == 0) {
+    q.out0[i] = make_float4(fx, fy, fz, 0.0f);
+  } else if constexpr (STAGE == 1) {
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (p.sd > 0.0f) {
+      float z0, z1, z2;
+      noise3(p.seed_lo, p.seed_hi, i - id0, *q.step, z0, z1, z2);
+      nx = p.sd * z0; ny = p.sd * z1; nz = p.sd * z2;
+    }
+    const float ex = rx + fx * p.dt + nx, ey = ry + fy * p.dt + ny, ez = rz + fz * p.dt + nz;
+    q.out0[i] = make_float4(ex, ey, ez, me.w);
+    q.pstar[i] = make_float4(rx + fx * p.half_dt + nx, ry + fy * p.half_dt + ny, rz + fz * p.half_dt + nz, 0.0f);
+    bin_bead(p, q, i, w, ex, ey, ez);
+  } else {
+    const float4 ps = q.pstar[i];
+    const float nx = ps.x + fx * p.half_dt, ny = ps.y + fy * p.half_dt, nz = ps.z + fz * p.half_dt;
+    q.out0[i] = make_float4(nx, ny, nz, me.w);
+    bin_bead(p, q, i, w, nx, ny, nz);
+  }
+}
+
+// ---------------------------------------------------------------- balanced force kernel
+// Eight lanes per bead (LANES = 8). The bead's neighbour rows (2R+1 planes of
+// 2R+1 rows) are trimmed to the cells that can hold a bead within its own
+// cutoff (the het-het one for het beads, the short one otherwise); the slots
+// of those cells are compacted into an LDS list, and the eight lanes take
+// every eighth entry. A bead's candidates are then spread evenly over its
+// lanes whatever the row lengths; three butterfly steps sum the parts.
+constexpr int C_LANES = 8;
+constexpr int C_BEADS_PER_BLOCK = (64 / C_LANES) * (BEAD_BLOCK / 64);
+constexpr int C_LIST = 128; // candidate slots per bead held in LDS
+
+// Cutoff of bead w's pairs, plus a margin for the rounding of cell edges.
+template <int MODE> __device__ __forceinline__ float trim_radius(const KParams &p, uint32_t w) {
+  float rc;
+  if constexpr (MODE == MODE_CCP) rc = (w & W_HET) ? sqrtf(p.aco2) : p.rco;
+  else if constexpr (MODE == MODE_ICG) rc = p.rco;
+  else rc = (w & W_HET) ? sqrtf(p.rc_lj2) : sqrtf(p.rm2);
+  return rc + 1e-3f;
+}
+
+// Cells [c0, c1) of row (cy + dy, cz + dz) that can hold a bead within rc of
+// (x, y, z): empty when the row's y-z slab is farther than rc, else the
+// cells met by the chord of half-length sqrt(rc^2 - gap^2), within reach R.
+template <int R>
+__device__ __forceinline__ void trimmed_row(const GridGeom &g, int cx, int cy, int cz, float x, float y, float z,
+                                            int dy, int dz, float rc, int &c0, int &c1) {
+  c0 = c1 = 0;
+  const int yy = cy + dy, zz = cz + dz;
+  if (yy < 0 || yy >= g.cps || zz < 0 || zz >= g.cps) return;
+  float gy = 0.0f, gz = 0.0f;
+  if (dy < 0) gy = y - (float)(yy - g.half + 1) * g.csl;
+  else if (dy > 0) gy = (float)(yy - g.half) * g.csl - y;
+  if (dz < 0) gz = z - (float)(zz - g.half + 1) * g.csl;
+  else if (dz > 0) gz = (float)(zz - g.half) * g.csl - z;
+  gy = fmaxf(gy, 0.0f);
+  gz = fmaxf(gz, 0.0f);
+  const float h2 = rc * rc - gy * gy - gz * gz;
+  if (h2 < 0.0f) return;
+  // a reciprocal is close enough here: the margin in rc keeps the range
+  // conservative by far more than its rounding
+  const float h = sqrtf(h2), ic = __builtin_amdgcn_rcpf(g.csl);
+  const int xl = max((int)floorf((x - h) * ic) + g.half, max(cx - R, 0));
+  const int xh = min((int)floorf((x + h) * ic) + g.half, min(cx + R, g.cps - 1));
+  if (xl > xh) return;
+  const int row = g.coff + (zz * g.cps + yy) * g.cps;
+  c0 = row + xl;
+  c1 = row + xh + 1;
+}
+
+template <int MODE, int STAGE, bool BATCH, int R>
+__global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
+  constexpr int W = 2 * R + 1, NROW = W * W, RPL = (NROW + C_LANES - 1) / C_LANES;
+  __shared__ uint32_t s_list[C_BEADS_PER_BLOCK][C_LIST];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const int sub = lane % C_LANES, grp = wave * (64 / C_LANES) + lane / C_LANES;
+  const int t = xcd_block() * C_BEADS_PER_BLOCK + grp;
+  const bool active = t < p.N;
+  const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const uint32_t w = wbits(me);
+  const uint32_t i = w & ID_MASK;
+  const float rx = me.x, ry = me.y, rz = me.z;
+  uint32_t id0 = 0;
+  if constexpr (BATCH) {
+    const SysParams &S = q.sys[system_of(q.sys_slot, q.n_sys, active ? (uint32_t)t : 0u)];
+    p.R_n = S.R_n; p.R_o = S.R_o; p.R_b = S.R_b;
+    p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
+    p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
+    p.seed_lo = S.seed_lo; p.seed_hi = S.seed_hi;
+    p.gf = S.g;
+    id0 = S.slot;
+  }
+  bool outside = false;
+  const int cx = cell_coord(rx, p.gf, outside), cy = cell_coord(ry, p.gf, outside),
+            cz = cell_coord(rz, p.gf, outside);
+  const float rc = trim_radius<MODE>(p, w);
+
+  // rows sub, sub + 8, ...: trimmed slot ranges and the LBS sites of LAD beads
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  uint32_t lo[RPL], len[RPL], tot = 0;
+#pragma unroll
+  for (int m = 0; m < RPL; ++m) {
+    const int r = sub + m * C_LANES;
+    int c0 = 0, c1 = 0;
+    if (active && r < NROW) trimmed_row<R>(p.gf, cx, cy, cz, rx, ry, rz, r % W - R, r / W - R, rc, c0, c1);
+    lo[m] = q.fcstart[c0];
+    len[m] = q.fcstart[c1] - lo[m];
+    tot += len[m];
+    if (!BATCH && MODE != MODE_ICG && p.n_l > 0 && (w & W_LAD)) {
+      const uint32_t llo = q.lstart[c0], lhi = q.lstart[c1];
+      for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
+    }
+  }
+  // exclusive prefix of the row lengths over the bead's eight lanes
+  uint32_t incl = tot;
+#pragma unroll
+  for (int o = 1; o < C_LANES; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o, C_LANES);
+    if (sub >= o) incl += v;
+  }
+  const uint32_t total = __shfl(incl, (lane & ~(C_LANES - 1)) + C_LANES - 1);
+  uint32_t off = incl - tot;
+  const bool listed = total <= (uint32_t)C_LIST;
+  if (listed) {
+#pragma unroll
+    for (int m = 0; m < RPL; ++m)
+      for (uint32_t k = 0; k < len[m]; ++k) s_list[grp][off++] = lo[m] + k;
+  }
+  // the list is read by the lanes of the same wavefront only: complete the
+  // LDS writes and order them before the reads, without a block barrier
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  if (active && sub == 0) {
+    bonded_force(p, q.pos, i, w, rx, ry, rz, fx, fy, fz);
+    wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
+  }
+  if (listed) {
+    uint32_t n = sub;
+    for (; n + C_LANES < total; n += 2 * C_LANES) {
+      const float4 a = q.fspos[s_list[grp][n]], b = q.fspos[s_list[grp][n + C_LANES]];
+      pair_full<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
+      pair_full<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
+    }
+    if (n < total) pair_full<MODE>(p, rx, ry, rz, w, q.fspos[s_list[grp][n]], fx, fy, fz);
+  } else {
+    // crowded neighbourhood: each lane scans its own rows
+#pragma unroll
+    for (int m = 0; m < RPL; ++m)
+      for (uint32_t k = 0; k < len[m]; ++k) pair_full<MODE>(p, rx, ry, rz, w, q.fspos[lo[m] + k], fx, fy, fz);
+  }
+#pragma unroll
+  for (int o = C_LANES / 2; o > 0; o >>= 1) {
+    fx += __shfl_xor(fx, o);
+    fy += __shfl_xor(fy, o);
+    fz += __shfl_xor(fz, o);
+  }
+  if (!active || sub != 0) return;
+  if (STAGE != 0 && p.fcap2 > 0.0f) {
+    const float m2 = fx * fx + fy * fy + fz * fz;
+    if (m2 > p.fcap2) {
+      const float sc = sqrtf(p.fcap2 / m2);
+      fx *= sc; fy *= sc; fz *= sc;
+    }
+  }
   if constexpr (STAGE == 0) {
     q.out0[i] = make_float4(fx, fy, fz, 0.0f);
   } else if constexpr (STAGE == 1) {
@@ -642,7 +822,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_bin_batch(const SysParams *sys, 
 // sequence, sugrid.cu:125-133). The block's bucket entries are staged in
 // LDS; each entry's rank within its cell (number of smaller bead indices)
 // is then found in parallel.
-constexpr int ORDER_LDS = 2048;
+constexpr int ORDER_LDS = 4096;
 
 __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, uint32_t *flags,
                                                       uint64_t *step_inc) {
@@ -957,35 +1137,55 @@ void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t 
   k_order<<<blocks, SCAN_BLOCK, 0, s>>>(a, jb, flags, step_inc);
 }
 
-template <int MODE, bool SPLIT, bool BATCH, int LANES>
+template <int MODE, bool SPLIT, bool BATCH, int R, int LANES>
 static void force_variant(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
   const unsigned g = blocks_for(p.N, Lanes<LANES>::BEADS_PER_BLOCK);
-  if (stage == 0) k_force<MODE, 0, SPLIT, BATCH, LANES><<<g, BEAD_BLOCK, 0, s>>>(p, q);
-  else if (stage == 1) k_force<MODE, 1, SPLIT, BATCH, LANES><<<g, BEAD_BLOCK, 0, s>>>(p, q);
-  else k_force<MODE, 2, SPLIT, BATCH, LANES><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  if (stage == 0) k_force<MODE, 0, SPLIT, BATCH, R, LANES><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else if (stage == 1) k_force<MODE, 1, SPLIT, BATCH, R, LANES><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else k_force<MODE, 2, SPLIT, BATCH, R, LANES><<<g, BEAD_BLOCK, 0, s>>>(p, q);
 }
 
-template <int MODE, bool SPLIT> static void force_mode(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
-  if (p.lanes == 1) force_variant<MODE, SPLIT, false, 1>(s, stage, p, q);
-  else force_variant<MODE, SPLIT, false, 3>(s, stage, p, q);
+template <int MODE, bool BATCH, int R>
+static void force_balanced(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
+  const unsigned g = blocks_for(p.N, C_BEADS_PER_BLOCK);
+  if (stage == 0) k_force_bal<MODE, 0, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else if (stage == 1) k_force_bal<MODE, 1, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  else k_force_bal<MODE, 2, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+}
+
+// kernel shape: cell reach 1 or 2; one lane per bead, one per plane, or
+// eight sharing a compacted candidate list
+template <int MODE, bool BATCH> static void force_shape(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
+  if (p.lanes == C_LANES) {
+    if (p.reach == 2) force_balanced<MODE, BATCH, 2>(s, stage, p, q);
+    else force_balanced<MODE, BATCH, 1>(s, stage, p, q);
+    return;
+  }
+  if (p.reach == 2) {
+    if (p.lanes == 1) force_variant<MODE, false, BATCH, 2, 1>(s, stage, p, q);
+    else force_variant<MODE, false, BATCH, 2, 5>(s, stage, p, q);
+  } else {
+    if (p.lanes == 1) force_variant<MODE, false, BATCH, 1, 1>(s, stage, p, q);
+    else force_variant<MODE, false, BATCH, 1, 3>(s, stage, p, q);
+  }
 }
 
 void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q) {
   if (p.N <= 0) return;
   if (q.sys) {
     // batches: the WCA/LJ set on one all-bead grid per system
-    if (p.lanes == 1) force_variant<MODE_LJ, false, true, 1>(s, stage, p, q);
-    else force_variant<MODE_LJ, false, true, 3>(s, stage, p, q);
+    force_shape<MODE_LJ, true>(s, stage, p, q);
     return;
   }
-  if (mode == MODE_ICG) force_mode<MODE_ICG, false>(s, stage, p, q);
-  else if (p.split) {
-    if (mode == MODE_CCP) force_mode<MODE_CCP, true>(s, stage, p, q);
-    else force_mode<MODE_LJ, true>(s, stage, p, q);
-  } else {
-    if (mode == MODE_CCP) force_mode<MODE_CCP, false>(s, stage, p, q);
-    else force_mode<MODE_LJ, false>(s, stage, p, q);
+  if (p.split && mode != MODE_ICG) {
+    // short pairs on the fine grid, het-het tail on the het grid (reach 1, a lane per plane)
+    if (mode == MODE_CCP) force_variant<MODE_CCP, true, false, 1, 3>(s, stage, p, q);
+    else force_variant<MODE_LJ, true, false, 1, 3>(s, stage, p, q);
+    return;
   }
+  if (mode == MODE_ICG) force_shape<MODE_ICG, false>(s, stage, p, q);
+  else if (mode == MODE_CCP) force_shape<MODE_CCP, false>(s, stage, p, q);
+  else force_shape<MODE_LJ, false>(s, stage, p, q);
 }
 
 void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map,

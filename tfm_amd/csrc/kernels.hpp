@@ -18,8 +18,9 @@ constexpr uint32_t W_B1 = 1u << 29; // r[i]   - r[i-1]
 constexpr uint32_t W_B2 = 1u << 30; // r[i+1] - r[i]
 constexpr uint32_t W_B3 = 1u << 31; // r[i+2] - r[i+1]
 
-// cells per scan block of the order kernel (one wavefront)
-constexpr int SCAN_BLOCK = 64;
+// cells per scan block of the order kernel; each block reads the totals of
+// the blocks before it, so fewer, larger blocks keep that prefix cheap
+constexpr int SCAN_BLOCK = 256;
 // threads per block of the bead kernels
 constexpr int BEAD_BLOCK = 256;
 
@@ -34,6 +35,7 @@ enum Mode : int { MODE_CCP = 0, MODE_ICG = 1, MODE_LJ = 2 };
 // the concatenated cell range of a batch (0 for a single system).
 struct GridGeom {
   float csl;
+  float icsl; // 1/csl for the cell lookup of the step grids; 0: divide by csl (ccp's rule, export grid)
   int cps, half, n_c, cap, nblk, coff;
 };
 
@@ -65,7 +67,8 @@ struct KParams {
   GridGeom gf; // grid of every bead (all pairs, or the short-range pass when split), LBS sites
   GridGeom gh; // grid of het beads (long-range het-het pass when split)
   int split;   // 1: short-range pairs on gf + het-het tail on gh; 0: every pair on gf
-  int lanes;   // lanes per bead of the force kernel: 3 (one per z-plane) or 1
+  int reach;   // neighbour cells scanned on each side: 1 (27 cells) or 2 (125 cells of half the side)
+  int lanes;   // lanes per bead of the force kernel: 2 reach + 1 (one per z-plane), 1, or 8 (balanced list)
   // nucleus geometry (cngeom, chrdat.cuh:48-69)
   float R_n, R_o, R_b, noc, boc, nod, bod;
   // bonds and bending

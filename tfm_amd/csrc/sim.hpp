@@ -105,6 +105,7 @@ struct ccp_sim {
   float contact_rc = 0;
   uint32_t cbins = 0, cbin_size = 0, cinterval = 0;
   uint32_t graph_steps = 64;
+  uint32_t reach = 1; // neighbour cells scanned on each side
   uint32_t lanes = 3; // lanes per bead of the force kernel
   float fcap = 0.0f;
   KParams kp{};
