@@ -222,7 +222,7 @@ def main():
             "overrides": args.param,
         },
         "roofline": {
-            "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "bound": "hbm", "limiter": "valu issue (see profiles/k_force_bal_lj_n131072_sq.txt)", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             # whole step at 176 algorithmic bytes per bead-step (index-array cell list)
             "step_frac_176B": 176.0 * value / (HBM_PEAK_GBS * 1e9),
