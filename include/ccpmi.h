@@ -235,6 +235,56 @@ int ccp_series_stats(const float *v, size_t n, double out[6]);
  * (sostat.cu:186-211): out = {avg, var, sem}. */
 int ccp_weighted_stats(const double *avg, const double *sem, size_t n, double out[3]);
 
+/* ---- batches of independent systems (no counterpart in ccp) ----
+ * Many systems with the WCA/LJ set, each with its own bead count, nucleus
+ * size, eps_het, eps_lam, het/LAD pattern and seed, stepped together: one
+ * force launch and one order launch per Heun stage cover all of them, and
+ * a hipGraph replays blocks of steps. Each system evolves exactly as a
+ * ccp_sim created from ccp_batch_system_params(b, s) would alone. Every
+ * contact_interval steps each system's contact map (contact_bins^2 uint32,
+ * ceil(n / contact_bins) beads per bin) and radial density (radial_bins
+ * equal-volume shells) are sampled. */
+typedef struct ccp_batch ccp_batch;
+
+typedef struct ccp_batch_system {
+  uint32_t n_beads;
+  uint32_t n_chains;      /* equal chains; 0 means 1 */
+  float volume_fraction;  /* sets the nucleus radius by ccp's formula (chrdat.cu:57-58) */
+  float eps_het;          /* het-het LJ well (kT) */
+  float eps_lam;          /* lamina well of LAD beads (kT) */
+  float het_fraction;
+  float lad_fraction;
+  uint64_t seed;          /* pattern, walk and noise key */
+} ccp_batch_system;
+
+/* common_params: parameter text shared by all systems (timestep,
+ * temperature, kernel shape, cell_capacity, contact_cutoff,
+ * contact_interval, contact_bins, radial_bins, het_block_mean,
+ * lad_placement, graph_steps, ...). NULL on error (ccp_batch_last_error(NULL)). */
+ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *systems, uint32_t n_sys);
+void ccp_batch_destroy(ccp_batch *b);
+const char *ccp_batch_last_error(const ccp_batch *b);
+/* offsets: n_sys + 1 first bead indices (system s holds beads [off[s], off[s+1])) */
+int ccp_batch_info(const ccp_batch *b, uint32_t *n_sys, uint32_t *n_beads, uint32_t *n_cells, uint32_t *offsets);
+/* the parameter text of system s alone (for a ccp_create of the same system) */
+int ccp_batch_system_params(const ccp_batch *b, uint32_t s, char *buf, size_t len);
+/* each system's initial condition as ccp_generate_initial_condition makes it */
+int ccp_batch_generate_initial_conditions(ccp_batch *b);
+int ccp_batch_run_steps(ccp_batch *b, uint64_t n);
+int ccp_batch_synchronize(ccp_batch *b);
+/* all beads: xyz (3 n floats) and CCP_HET/CCP_LAD flags (n bytes); either may be NULL */
+int ccp_batch_get_state(ccp_batch *b, float *xyz, uint8_t *flags, size_t n);
+int ccp_batch_set_state(ccp_batch *b, const float *xyz, const uint8_t *flags, size_t n);
+int ccp_batch_get_step(const ccp_batch *b, uint64_t *step);
+int ccp_batch_set_step(ccp_batch *b, uint64_t step);
+/* n = n_sys * contact_bins^2 summed counts; n_samples = samples taken */
+int ccp_batch_contact_maps(ccp_batch *b, uint32_t *maps, size_t n, uint32_t *n_samples);
+/* n = n_sys * radial_bins mean bead volume fraction per shell */
+int ccp_batch_radial_density(ccp_batch *b, double *rd, size_t n);
+int ccp_batch_reset_samples(ccp_batch *b);
+/* device time per step of n eager steps (HIP events), for the autotuner */
+int ccp_batch_time_steps(ccp_batch *b, uint32_t n, float *ms_per_step);
+
 /* Average device time per launch (ms) of each kernel of the step over
  * n_steps steps, measured with HIP events on the handle's stream:
  * out[0] force stage 1, [1] order (scan + sort + gather) after stage 1,

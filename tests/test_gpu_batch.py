@@ -1,0 +1,95 @@
+"""GPU tests of the batch engine: every system of a batch evolves bit for bit
+as the same system stepped alone (same cells, pair order and noise
+counters), its sampled contact map equals the single run's, and its radial
+density conserves the bead volume."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import tfm_amd
+from gpu_common import step
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = [
+    dict(n_beads=600, volume_fraction=0.15, eps_het=0.3, eps_lam=0.5, het_fraction=0.3, lad_fraction=0.1, seed=4001),
+    dict(n_beads=1000, volume_fraction=0.3, eps_het=0.9, eps_lam=1.4, het_fraction=0.3, lad_fraction=0.1, seed=4002),
+    dict(n_beads=2048, volume_fraction=0.2, eps_het=0.6, eps_lam=1.0, het_fraction=0.3, lad_fraction=0.1, seed=4003),
+    dict(n_beads=777, volume_fraction=0.35, eps_het=0.2, eps_lam=0.2, het_fraction=0.3, lad_fraction=0.1, seed=4004),
+    dict(n_beads=1500, volume_fraction=0.1, eps_het=1.0, eps_lam=1.5, het_fraction=0.3, lad_fraction=0.1, seed=4005,
+         n_chains=2),
+]
+
+
+def single_from(batch, s, xyz, fl):
+    sim = tfm_amd.Simulation(batch.system_params(s))
+    o0, o1 = batch.offsets[s], batch.offsets[s + 1]
+    sim.flags = fl[o0:o1]
+    sim.positions = xyz[o0:o1]
+    sim.set_time(0, 0.0, 0)
+    return sim
+
+
+@pytest.mark.parametrize("shape", [dict(), dict(lanes_per_bead=8, cell_reach=2, cell_size=1.25)])
+def test_batch_systems_match_single_runs(shape):
+    step(f"batch of {len(SYSTEMS)} systems, shape {shape}")
+    common = dict(timestep=0.001, contact_interval=100, contact_cutoff=1.5, graph_steps=50, **shape)
+    b = tfm_amd.Batch(SYSTEMS, common)
+    assert b.N == sum(d["n_beads"] for d in SYSTEMS)
+    b.generate_initial_conditions()
+    xyz0, fl0 = b.state()
+    b.run_steps(300)
+    b.synchronize()
+    xyz, _ = b.state()
+    maps, ns = b.contact_maps()
+    assert ns == 3 and b.step == 300
+    for s in range(len(SYSTEMS)):
+        sim = single_from(b, s, xyz0, fl0)
+        sim.run_steps(300)
+        sim.synchronize()
+        o0, o1 = b.offsets[s], b.offsets[s + 1]
+        assert np.array_equal(sim.positions, xyz[o0:o1]), s
+        m, n1 = sim.contact_map()
+        k = m.shape[0]  # the system alone keeps ceil(n / bin size) bins; the batch pads to 128
+        assert n1 == 3 and np.array_equal(m, maps[s][:k, :k]), s
+        assert maps[s][k:].sum() == 0 and maps[s][:, k:].sum() == 0
+        sim.close()
+    assert np.isfinite(xyz).all()
+    b.close()
+
+
+def test_batch_initial_conditions_and_radial_density():
+    step("batch initial conditions, radial density")
+    b = tfm_amd.Batch(SYSTEMS, dict(timestep=0.001, contact_interval=50, graph_steps=25))
+    b.generate_initial_conditions()
+    xyz, fl = b.state()
+    R = []
+    for s, d in enumerate(SYSTEMS):
+        o0, o1 = b.offsets[s], b.offsets[s + 1]
+        # the same initial condition as the system alone
+        sim = tfm_amd.Simulation(b.system_params(s))
+        sim.generate_initial_condition()
+        assert np.array_equal(sim.positions, xyz[o0:o1]) and np.array_equal(sim.flags, fl[o0:o1])
+        R.append((float(sim.info.R_n), float(sim.info.R_o)))
+        assert np.linalg.norm(xyz[o0:o1].astype(np.float64), axis=1).max() < R[-1][0]
+        sim.close()
+    # four samples, each checked against the oracle's lens-volume density
+    want = np.zeros((len(SYSTEMS), 32))
+    for _ in range(4):
+        b.run_steps(50)
+        xyz, fl = b.state()
+        for s in range(len(SYSTEMS)):
+            o0, o1 = b.offsets[s], b.offsets[s + 1]
+            R_n, R_o = R[s]
+            nod = float(np.float32(np.sqrt(R_n * R_n - R_o * R_o)))
+            _, rcd = po.observables(xyz[o0:o1].astype(np.float64), fl[o0:o1], R_n, nod=nod, n_b=32)
+            want[s] += rcd[2]
+    rd = b.radial_density()
+    assert rd.shape == (len(SYSTEMS), 32)
+    np.testing.assert_allclose(rd, want / 4, rtol=1e-9, atol=1e-12)
+    b.close()
+
+
+def test_batch_rejects_bad_systems():
+    with pytest.raises(tfm_amd.CcpError, match="out of range"):
+        tfm_amd.Batch([dict(n_beads=100, volume_fraction=0.9, seed=1)])

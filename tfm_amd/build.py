@@ -31,7 +31,7 @@ def _run(cmd):
 
 
 def build_lib():
-    srcs = [os.path.join(CSRC, "kernels.hip"), os.path.join(CSRC, "ccpmi.cpp")]
+    srcs = [os.path.join(CSRC, "kernels.hip"), os.path.join(CSRC, "ccpmi.cpp"), os.path.join(CSRC, "batch.cpp")]
     tmp = LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, LIB)

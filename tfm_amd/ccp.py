@@ -30,6 +30,10 @@ EXPORTS = (
     "ccp_distance_contact", "ccp_frame_contact_map", "ccp_analysis_create", "ccp_analysis_destroy",
     "ccp_analysis_add_frame", "ccp_analysis_add_trajectory_file", "ccp_analysis_end_simulation",
     "ccp_analysis_finish", "ccp_analysis_last_error", "ccp_series_stats", "ccp_weighted_stats",
+    "ccp_batch_create", "ccp_batch_destroy", "ccp_batch_last_error", "ccp_batch_info", "ccp_batch_system_params",
+    "ccp_batch_generate_initial_conditions", "ccp_batch_run_steps", "ccp_batch_synchronize", "ccp_batch_get_state",
+    "ccp_batch_set_state", "ccp_batch_get_step", "ccp_batch_set_step", "ccp_batch_contact_maps",
+    "ccp_batch_radial_density", "ccp_batch_reset_samples", "ccp_batch_time_steps",
 )
 
 
@@ -49,6 +53,12 @@ class Info(ctypes.Structure):
 
 
 LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
+
+
+class BatchSystem(ctypes.Structure):
+    _fields_ = [("n_beads", ctypes.c_uint32), ("n_chains", ctypes.c_uint32), ("volume_fraction", ctypes.c_float),
+                ("eps_het", ctypes.c_float), ("eps_lam", ctypes.c_float), ("het_fraction", ctypes.c_float),
+                ("lad_fraction", ctypes.c_float), ("seed", ctypes.c_uint64)]
 
 # ccp's chromosome limits at N = 30386 (chrdat.cuh:27-29)
 _CHR_LIM = (0, 2284, 4599, 6687, 9338, 12508, 15193, 17477, 19792, 21880, 24531, 27701, 30386)
@@ -112,6 +122,23 @@ def lib():
         L.ccp_analysis_last_error.restype = ctypes.c_char_p
         L.ccp_series_stats.argtypes = [f32p, sz, f64p]
         L.ccp_weighted_stats.argtypes = [f64p, f64p, sz, f64p]
+        L.ccp_batch_create.argtypes = [ctypes.c_char_p, P(BatchSystem), ctypes.c_uint32]
+        L.ccp_batch_create.restype = vp
+        L.ccp_batch_destroy.argtypes = [vp]
+        L.ccp_batch_last_error.argtypes = [vp]
+        L.ccp_batch_last_error.restype = ctypes.c_char_p
+        L.ccp_batch_info.argtypes = [vp, u32p, u32p, u32p, u32p]
+        L.ccp_batch_system_params.argtypes = [vp, ctypes.c_uint32, ctypes.c_char_p, sz]
+        for n in ("ccp_batch_generate_initial_conditions", "ccp_batch_synchronize", "ccp_batch_reset_samples"):
+            getattr(L, n).argtypes = [vp]
+        L.ccp_batch_run_steps.argtypes = [vp, ctypes.c_uint64]
+        L.ccp_batch_get_state.argtypes = [vp, f32p, u8p, sz]
+        L.ccp_batch_set_state.argtypes = [vp, f32p, u8p, sz]
+        L.ccp_batch_get_step.argtypes = [vp, P(ctypes.c_uint64)]
+        L.ccp_batch_set_step.argtypes = [vp, ctypes.c_uint64]
+        L.ccp_batch_contact_maps.argtypes = [vp, u32p, sz, u32p]
+        L.ccp_batch_radial_density.argtypes = [vp, f64p, sz]
+        L.ccp_batch_time_steps.argtypes = [vp, ctypes.c_uint32, f32p]
         _lib = L
     return _lib
 
@@ -388,3 +415,102 @@ def weighted_stats(avg, sem):
     if rc:
         raise CcpError(rc, "bad input")
     return out
+
+
+class Batch:
+    """Independent systems of the WCA/LJ set stepped together on one GPU.
+
+    systems: list of dicts with n_beads, volume_fraction, eps_het, eps_lam,
+    het_fraction, lad_fraction, seed (and optionally n_chains)."""
+
+    def __init__(self, systems, common=None):
+        self.h = None
+        arr = (BatchSystem * len(systems))()
+        for a, d in zip(arr, systems):
+            a.n_beads = int(d["n_beads"])
+            a.n_chains = int(d.get("n_chains", 1))
+            a.volume_fraction = float(d["volume_fraction"])
+            a.eps_het = float(d.get("eps_het", 0.5))
+            a.eps_lam = float(d.get("eps_lam", 1.0))
+            a.het_fraction = float(d.get("het_fraction", 0.3))
+            a.lad_fraction = float(d.get("lad_fraction", 0.1))
+            a.seed = int(d["seed"])
+        h = lib().ccp_batch_create(params_text(common or {}).encode(), arr, len(systems))
+        if not h:
+            raise CcpError(1, lib().ccp_batch_last_error(None).decode())
+        self.h = h
+        ns, nb, nc = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        off = np.zeros(len(systems) + 1, np.uint32)
+        lib().ccp_batch_info(h, ctypes.byref(ns), ctypes.byref(nb), ctypes.byref(nc), _ptr(off, ctypes.c_uint32))
+        self.n_sys, self.N, self.n_cells = ns.value, nb.value, nc.value
+        self.offsets = off
+        common = common or {}
+        self.bins = int(common.get("contact_bins", 128))
+        self.rd_bins = int(common.get("radial_bins", 32))
+
+    def close(self):
+        if self.h:
+            lib().ccp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CcpError(rc, lib().ccp_batch_last_error(self.h).decode())
+
+    def system_params(self, s):
+        buf = ctypes.create_string_buffer(4096)
+        self._check(lib().ccp_batch_system_params(self.h, s, buf, 4096))
+        return buf.value.decode()
+
+    def generate_initial_conditions(self):
+        self._check(lib().ccp_batch_generate_initial_conditions(self.h))
+
+    def run_steps(self, n):
+        self._check(lib().ccp_batch_run_steps(self.h, n))
+
+    def synchronize(self):
+        self._check(lib().ccp_batch_synchronize(self.h))
+
+    def state(self):
+        xyz = np.zeros((self.N, 3), np.float32)
+        fl = np.zeros(self.N, np.uint8)
+        self._check(lib().ccp_batch_get_state(self.h, _ptr(xyz, ctypes.c_float), _ptr(fl, ctypes.c_uint8), self.N))
+        return xyz, fl
+
+    def set_state(self, xyz, flags):
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        flags = np.ascontiguousarray(flags, np.uint8)
+        self._check(lib().ccp_batch_set_state(self.h, _ptr(xyz, ctypes.c_float), _ptr(flags, ctypes.c_uint8),
+                                              self.N))
+
+    @property
+    def step(self):
+        v = ctypes.c_uint64()
+        self._check(lib().ccp_batch_get_step(self.h, ctypes.byref(v)))
+        return v.value
+
+    @step.setter
+    def step(self, value):
+        self._check(lib().ccp_batch_set_step(self.h, int(value)))
+
+    def contact_maps(self):
+        m = np.zeros((self.n_sys, self.bins, self.bins), np.uint32)
+        ns = ctypes.c_uint32()
+        self._check(lib().ccp_batch_contact_maps(self.h, _ptr(m, ctypes.c_uint32), m.size, ctypes.byref(ns)))
+        return m, ns.value
+
+    def radial_density(self):
+        rd = np.zeros((self.n_sys, self.rd_bins))
+        self._check(lib().ccp_batch_radial_density(self.h, _ptr(rd, ctypes.c_double), rd.size))
+        return rd
+
+    def reset_samples(self):
+        self._check(lib().ccp_batch_reset_samples(self.h))
+
+    def time_steps(self, n):
+        ms = ctypes.c_float()
+        self._check(lib().ccp_batch_time_steps(self.h, n, ctypes.byref(ms)))
+        return ms.value
