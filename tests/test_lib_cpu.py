@@ -11,18 +11,24 @@ from tfm_amd import ccp
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "ccpmi.h")).read()
+def declared_symbols(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return set(re.findall(r"\b(ccp_[a-z_]+)\s*\(", text))
+    return set(re.findall(r"\b(ccp_[a-z0-9_]+)\s*\(", text))
 
 
 def test_library_exports_every_declared_symbol():
+    from tfm_amd import surrogate
+
     L = ccp.lib()
-    syms = declared_symbols()
+    syms = declared_symbols("ccpmi.h")
     assert len(syms) >= 30
     assert syms == set(ccp.EXPORTS)
-    for s in syms:
+    nn = declared_symbols("ccpmi_nn.h")
+    assert nn == set(surrogate.NN_EXPORTS)
+    headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert headers == ["ccpmi.h", "ccpmi_nn.h"]
+    for s in syms | nn:
         assert hasattr(L, s), s
 
 
