@@ -227,6 +227,12 @@ int ccp_analysis_finish(ccp_analysis *a, const char *fin_path, const char *cm_pa
 /* Error message of the last failed call on a. */
 const char *ccp_analysis_last_error(const ccp_analysis *a);
 
+/* The bead flags (CCP_HET, CCP_LAD) that ccp_generate_initial_condition of
+ * a simulation with these parameters would assign (set_particle_types,
+ * chrsim.cu:634-675, or the het/LAD generator), without creating the
+ * simulation or touching a GPU. Errors: ccp_last_error(NULL). */
+int ccp_sequence_pattern(const char *param_text, uint8_t *flags, size_t n);
+
 /* ccp's time-series statistics of n float values (calc_stats for tdstat,
  * sostat.cu:141-183): out = {avg, var, sem, f_n_b, i_t, thermalised}. */
 int ccp_series_stats(const float *v, size_t n, double out[6]);
@@ -255,6 +261,8 @@ typedef struct ccp_batch_system {
   float het_fraction;
   float lad_fraction;
   uint64_t seed;          /* pattern, walk and noise key */
+  uint32_t replica;       /* 0, or k > 0: same pattern as replica 0, its own walk and noise */
+  uint32_t pad;
 } ccp_batch_system;
 
 /* common_params: parameter text shared by all systems (timestep,

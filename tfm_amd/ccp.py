@@ -33,7 +33,7 @@ EXPORTS = (
     "ccp_batch_create", "ccp_batch_destroy", "ccp_batch_last_error", "ccp_batch_info", "ccp_batch_system_params",
     "ccp_batch_generate_initial_conditions", "ccp_batch_run_steps", "ccp_batch_synchronize", "ccp_batch_get_state",
     "ccp_batch_set_state", "ccp_batch_get_step", "ccp_batch_set_step", "ccp_batch_contact_maps",
-    "ccp_batch_radial_density", "ccp_batch_reset_samples", "ccp_batch_time_steps",
+    "ccp_batch_radial_density", "ccp_batch_reset_samples", "ccp_batch_time_steps", "ccp_sequence_pattern",
 )
 
 
@@ -58,7 +58,8 @@ LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
 class BatchSystem(ctypes.Structure):
     _fields_ = [("n_beads", ctypes.c_uint32), ("n_chains", ctypes.c_uint32), ("volume_fraction", ctypes.c_float),
                 ("eps_het", ctypes.c_float), ("eps_lam", ctypes.c_float), ("het_fraction", ctypes.c_float),
-                ("lad_fraction", ctypes.c_float), ("seed", ctypes.c_uint64)]
+                ("lad_fraction", ctypes.c_float), ("seed", ctypes.c_uint64), ("replica", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
 
 # ccp's chromosome limits at N = 30386 (chrdat.cuh:27-29)
 _CHR_LIM = (0, 2284, 4599, 6687, 9338, 12508, 15193, 17477, 19792, 21880, 24531, 27701, 30386)
@@ -139,6 +140,7 @@ def lib():
         L.ccp_batch_contact_maps.argtypes = [vp, u32p, sz, u32p]
         L.ccp_batch_radial_density.argtypes = [vp, f64p, sz]
         L.ccp_batch_time_steps.argtypes = [vp, ctypes.c_uint32, f32p]
+        L.ccp_sequence_pattern.argtypes = [ctypes.c_char_p, u8p, sz]
         _lib = L
     return _lib
 
@@ -435,6 +437,7 @@ class Batch:
             a.het_fraction = float(d.get("het_fraction", 0.3))
             a.lad_fraction = float(d.get("lad_fraction", 0.1))
             a.seed = int(d["seed"])
+            a.replica = int(d.get("replica", 0))
         h = lib().ccp_batch_create(params_text(common or {}).encode(), arr, len(systems))
         if not h:
             raise CcpError(1, lib().ccp_batch_last_error(None).decode())
@@ -514,3 +517,12 @@ class Batch:
         ms = ctypes.c_float()
         self._check(lib().ccp_batch_time_steps(self.h, n, ctypes.byref(ms)))
         return ms.value
+
+
+def sequence_pattern(params, n):
+    """Bead flags the initial-condition generator assigns for these parameters (host only)."""
+    fl = np.zeros(n, np.uint8)
+    rc = lib().ccp_sequence_pattern(params_text(params).encode(), _ptr(fl, ctypes.c_uint8), n)
+    if rc:
+        raise CcpError(rc, lib().ccp_last_error(None).decode())
+    return fl

@@ -296,10 +296,10 @@ std::string system_text(const std::string &common, const ccp_batch_system &s, ui
   std::snprintf(buf, sizeof(buf),
                 "number_of_particles %u\nnucleus_radius %.6f\nnumber_of_chains %u\nforce_field lj\n"
                 "epsilon_het %.9g\nepsilon_lamina %.9g\nhet_fraction %.9g\nlad_fraction %.9g\nseed %llu\n"
-                "number_of_lbs 0\ncontact_bins %u\ncontact_bin_size %u\n",
+                "number_of_lbs 0\ncontact_bins %u\ncontact_bin_size %u\nreplica %u\n",
                 s.n_beads, R_n, s.n_chains ? s.n_chains : 1u, (double)s.eps_het, (double)s.eps_lam,
                 (double)s.het_fraction, (double)s.lad_fraction, (unsigned long long)s.seed, bins,
-                (s.n_beads + bins - 1) / bins);
+                (s.n_beads + bins - 1) / bins, s.replica);
   return common + "\n" + buf;
 }
 
@@ -337,8 +337,8 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
       S.R_n = one.R_n; S.R_o = one.R_o; S.R_b = one.R_b;
       S.noc = one.noc; S.boc = one.boc; S.nod = one.nod; S.bod = one.bod;
       S.eps_het = one.eps_het; S.eps_lam = one.eps_lam;
-      S.seed_lo = (uint32_t)one.seed;
-      S.seed_hi = (uint32_t)(one.seed >> 32);
+      S.seed_lo = one.kp.seed_lo;
+      S.seed_hi = one.kp.seed_hi;
       S.slot = (uint32_t)total;
       S.n = one.N;
       S.g = one.gf.g;

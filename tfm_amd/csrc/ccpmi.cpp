@@ -71,6 +71,8 @@ void ccp_sim::configure(const ParMap &par) {
 
   // ---- build-defined keys
   seed = par.has("seed") ? par.get<uint64_t>("seed", 0) : (uint64_t)time(nullptr);
+  replica = par.get<uint32_t>("replica", 0u);
+  if (replica >= (1u << 15)) throw Error(CCP_ERR_PARAM, "replica out of range");
   std::string ffs = par.get<std::string>("force_field", "ccp");
   if (ffs == "ccp") ff = CCP_FF_CCP;
   else if (ffs == "lj") ff = CCP_FF_LJ;
@@ -228,8 +230,10 @@ void ccp_sim::fill_kparams() {
   k.half_dt = 0.5f * dt;
   k.sd = sd;
   k.fcap2 = fcap * fcap;
-  k.seed_lo = (uint32_t)seed;
-  k.seed_hi = (uint32_t)(seed >> 32);
+  // replicas > 0 draw their noise with another key (golden-ratio offset)
+  const uint64_t key = seed + (uint64_t)replica * 0x9E3779B97F4A7C15ull;
+  k.seed_lo = (uint32_t)key;
+  k.seed_hi = (uint32_t)(key >> 32);
   k.contact_rc = contact_rc;
   k.contact_bin_size = (int)cbin_size;
   k.contact_bins = (int)cbins;
@@ -670,7 +674,13 @@ void ccp_sim::generate_initial_condition() {
   HostRng g(seed);
   set_lbs_positions(g);
   set_particle_types(g);
-  set_particle_positions(g);
+  if (replica) {
+    // the same sites and types as replica 0, an independent walk
+    HostRng w(seed, replica);
+    set_particle_positions(w);
+  } else {
+    set_particle_positions(g);
+  }
   upload_lbs();
   upload_positions();
   uint64_t po = count_overlaps(MIS), ipo = po;
@@ -1560,6 +1570,28 @@ int ccp_analysis_finish(ccp_analysis *a, const char *fin_path, const char *cm_pa
     b.write(reinterpret_cast<const char *>(m1.data()), (std::streamsize)m1.size() * 8);
     if (b.fail()) throw Error(CCP_ERR_IO, std::string("failed to write ") + cm_path);
   });
+}
+
+int ccp_sequence_pattern(const char *param_text, uint8_t *flags, size_t n) {
+  try {
+    ccp_sim one;
+    one.log_fn = [](const char *, void *) {};
+    ParMap par(param_text);
+    one.configure(par);
+    if (!flags || n != one.N) throw Error(CCP_ERR_ARG, "pattern size mismatch");
+    // the draws of generate_initial_condition up to the bead types
+    HostRng g(one.seed);
+    one.set_lbs_positions(g);
+    one.set_particle_types(g);
+    std::memcpy(flags, one.hfl.data(), n);
+    return CCP_OK;
+  } catch (const Error &e) {
+    g_create_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_create_error = e.what();
+    return CCP_ERR_ARG;
+  }
 }
 
 int ccp_series_stats(const float *v, size_t n, double out[6]) {

@@ -27,9 +27,11 @@ struct HostRng {
   uint64_t n = 0;
   uint32_t buf[4];
   int pos = 4;
-  explicit HostRng(uint64_t seed) : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)) {}
+  uint32_t stream = 1u << 16; // fourth counter word; replicas > 0 use their own stream
+  explicit HostRng(uint64_t seed, uint32_t replica = 0)
+      : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), stream((1u << 16) + replica) {}
   void refill() {
-    uint32_t c[4] = {(uint32_t)n, (uint32_t)(n >> 32), 0xffffffffu, 1u << 16};
+    uint32_t c[4] = {(uint32_t)n, (uint32_t)(n >> 32), 0xffffffffu, stream};
     uint32_t a = k0, b = k1;
     for (int r = 0; r < 10; ++r) {
       uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
@@ -90,6 +92,7 @@ struct ccp_sim {
   float noc = 0, boc = 0, nod = 0, bod = 0, d_m = 0;
   float dt = 0, sd = 0;
   uint64_t seed = 0;
+  uint32_t replica = 0; // > 0: same bead types as replica 0, its own walk and noise
   int ff = CCP_FF_CCP;
   float csl = 0;   // coarse cell side: het grid and ccp-layout export grid
   float csl_f = 0; // cell side of the all-bead grid of the step (fine when split)

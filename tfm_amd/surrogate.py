@@ -139,8 +139,14 @@ class Surrogate:
             _ok(L.ccp_nn_cast_bf16(_p(self.p[k]), _p(t), t.numel(), _stream()), "cast")
 
     # ------------------------------------------------------------ forward
+    @staticmethod
+    def _check_input(t, shape, name):
+        if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{name} must be a contiguous float32 cuda tensor of shape {tuple(shape)}")
+
     def forward(self, x):
         """x: [c_in, B L] float32 (cuda). Returns (S [B, L, L], rd [rd_bins, B]) on the device."""
+        self._check_input(x, (self.c_in, self.B * self.L), "x")
         N = _nn()
         B, L, W, d = self.B, self.L, self.width, self.d
         P = B * L
@@ -181,6 +187,8 @@ class Surrogate:
         B, L, W, d = self.B, self.L, self.width, self.d
         P = B * L
         s = _stream()
+        self._check_input(T, (B, L, L), "T")
+        self._check_input(T_rd, (self.rd_bins, B), "T_rd")
         self.forward(x)
         self.loss.zero_()
         _ok(N.ccp_nn_pair_loss(_p(self.S), _p(T), _p(self.gS), _p(self.graw), _p(self.grawb), _p(self.g["dist"]),
