@@ -93,3 +93,32 @@ def test_batch_initial_conditions_and_radial_density():
 def test_batch_rejects_bad_systems():
     with pytest.raises(tfm_amd.CcpError, match="out of range"):
         tfm_amd.Batch([dict(n_beads=100, volume_fraction=0.9, seed=1)])
+
+
+def test_replicas_share_the_pattern_not_the_walk():
+    step("batch replicas")
+    base = dict(n_beads=900, volume_fraction=0.2, eps_het=0.5, eps_lam=1.0, seed=77)
+    b = tfm_amd.Batch([dict(base, replica=0), dict(base, replica=1), dict(base, replica=2)],
+                      dict(timestep=0.001, contact_interval=0))
+    b.generate_initial_conditions()
+    xyz, fl = b.state()
+    o = b.offsets
+    assert np.array_equal(fl[o[0]:o[1]], fl[o[1]:o[2]]) and np.array_equal(fl[o[0]:o[1]], fl[o[2]:o[3]])
+    assert not np.array_equal(xyz[o[0]:o[1]], xyz[o[1]:o[2]])
+    # replica 0 is the plain simulation; the host-only pattern matches its types
+    sim = tfm_amd.Simulation(b.system_params(0))
+    sim.generate_initial_condition()
+    assert np.array_equal(sim.positions, xyz[o[0]:o[1]])
+    assert np.array_equal(tfm_amd.ccp.sequence_pattern(b.system_params(0), 900), sim.flags)
+    sim.close()
+    # replica 1 alone evolves as in the batch (its own noise key)
+    b.run_steps(200)
+    xyz2, _ = b.state()
+    one = tfm_amd.Simulation(b.system_params(1))
+    one.flags = fl[o[1]:o[2]]
+    one.positions = xyz[o[1]:o[2]]
+    one.set_time(0, 0.0, 0)
+    one.run_steps(200)
+    assert np.array_equal(one.positions, xyz2[o[1]:o[2]])
+    one.close()
+    b.close()

@@ -3,6 +3,7 @@ parameters with ccp's messages before touching a GPU."""
 import os
 import re
 
+import numpy as np
 import pytest
 
 import tfm_amd
@@ -75,3 +76,19 @@ def test_first_occurrence_of_a_key_wins():
     # parmap inserts with std::map::insert, keeping the first value (util.cu:72-80)
     with pytest.raises(tfm_amd.CcpError, match="nucleus_radius out of range"):
         tfm_amd.Simulation("number_of_particles 1024\nnucleus_radius 0\nnucleus_radius 10.5\n")
+
+
+def test_sequence_pattern_is_host_only_and_deterministic():
+    from tfm_amd.ccp import sequence_pattern
+
+    p = {"number_of_particles": 4096, "nucleus_radius": "15.0", "force_field": "lj", "het_fraction": 0.3,
+         "lad_fraction": 0.15, "seed": 11, "number_of_chains": 2}
+    a = sequence_pattern(p, 4096)
+    assert np.array_equal(a, sequence_pattern(p, 4096))
+    assert abs(((a & 1) > 0).mean() - 0.3) < 0.06
+    assert ((a & 2) > 0).sum() == 2 * round(0.15 * 2048)
+    assert not np.array_equal(a, sequence_pattern({**p, "seed": 12}, 4096))
+    # replicas keep the pattern
+    assert np.array_equal(a, sequence_pattern({**p, "replica": 3}, 4096))
+    with pytest.raises(tfm_amd.CcpError):
+        sequence_pattern(p, 100)
