@@ -189,6 +189,11 @@ def one_by_one_rate(draws, n_beads, warmup, steps, interval, shape, log=print):
 
 
 def main(argv=None):
+    print(json.dumps(run(argv)), flush=True)
+
+
+def run(argv=None):
+    """The whole sweep; returns the result record."""
     ap = argparse.ArgumentParser()
     ap.add_argument("--n-beads", type=int, default=2048)
     ap.add_argument("--train", type=int, default=4096)
@@ -272,7 +277,7 @@ def main(argv=None):
         "seconds": {"simulate_train_sets": t_sim, "train": t_train, "candidate_features": t_feat,
                     "score": t_score, "verify": t_verify, "total": t_total},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
