@@ -122,3 +122,14 @@ def test_replicas_share_the_pattern_not_the_walk():
     assert np.array_equal(one.positions, xyz2[o[1]:o[2]])
     one.close()
     b.close()
+
+
+def test_batch_autotuner_picks_a_measured_shape():
+    step("batch autotuner")
+    from tfm_amd import autotune
+
+    systems = [dict(n_beads=int(n), volume_fraction=0.25, eps_het=0.6, eps_lam=1.0, seed=900 + k)
+               for k, n in enumerate(np.geomspace(512, 4096, 12).astype(int))]
+    best, table = autotune.tune_batch(systems, dict(timestep=0.001), steps=50, warmup=10)
+    assert len(table) == 6 and all(np.isfinite(t) and t > 0 for _, t in table)
+    assert best == min(table, key=lambda e: e[1])[0]

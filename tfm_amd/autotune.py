@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-from .ccp import Simulation
+from .ccp import Batch, Simulation
 
 
 def candidates(cutoff):
@@ -76,3 +76,28 @@ def tune(params, positions, flags, cutoff, steps=200, warmup=50, cache=None, log
         with open(cache, "w") as f:
             json.dump(data, f, indent=1)
     return best, table
+
+
+def tune_batch(systems, common, cutoff=2.5, steps=100, warmup=20, log=None):
+    """Time every candidate shape on a batch of systems (tfm_amd.Batch) from
+    the same initial conditions; returns (best overrides, [(overrides, ms per
+    step), ...])."""
+    ref = Batch(systems, common)
+    try:
+        ref.generate_initial_conditions()
+        xyz, fl = ref.state()
+    finally:
+        ref.close()
+    table = []
+    for cand in candidates(cutoff):
+        b = Batch(systems, {**common, **cand})
+        try:
+            b.set_state(xyz, fl)
+            b.run_steps(warmup)
+            ms = b.time_steps(steps)
+        finally:
+            b.close()
+        table.append((cand, ms))
+        if log:
+            log(f"[autotune] batch {cand} {ms * 1e3:.1f} us/step")
+    return min(table, key=lambda e: e[1])[0], table
