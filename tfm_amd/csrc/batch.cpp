@@ -57,6 +57,7 @@ struct BatchGrid {
 
 struct ccp_batch {
   uint32_t n_sys = 0, N = 0, bins = 128, rd_bins = 32, cinterval = 0, graph_steps = 64;
+  bool integrate_pass = false;
   std::vector<std::string> text;     // parameter text of each system alone
   std::vector<SysParams> sys;
   std::vector<uint32_t> slot;        // n_sys + 1
@@ -73,7 +74,7 @@ struct ccp_batch {
   // device
   int dev = 0;
   hipStream_t stream = nullptr;
-  DevBuf<float4> r, er, pstar;
+  DevBuf<float4> r, er, pstar, fpair;
   DevBuf<SysParams> dsys;
   DevBuf<uint32_t> dslot, flags, cmaps;
   DevBuf<uint64_t> dstep;
@@ -91,7 +92,7 @@ struct ccp_batch {
   }
   ~ccp_batch() {
     drop_graphs();
-    DevBuf<float4> *f4[] = {&r, &er, &pstar};
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &fpair};
     for (auto *b : f4) b->free_();
     DevBuf<uint32_t> *u4[] = {&dslot, &flags, &cmaps};
     for (auto *b : u4) b->free_();
@@ -188,6 +189,7 @@ struct ccp_batch {
     q.fbucket = grid.bucket.p;
     q.flags = flags.p;
     q.step = dstep.p;
+    q.fpair = fpair.p;
     q.sys = dsys.p;
     q.sys_slot = dslot.p;
     q.n_sys = (int)n_sys;
@@ -348,6 +350,7 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
         b->contact_rc = one.contact_rc;
         b->cinterval = one.cinterval;
         b->graph_steps = one.graph_steps;
+        b->integrate_pass = one.integrate_pass;
       } else if (one.kp.lanes != b->kp.lanes || one.kp.reach != b->kp.reach || one.gf.g.csl != b->kp.gf.csl ||
                  one.gf.g.cap != b->kp.gf.cap || one.kp.dt != b->kp.dt || one.kp.sd != b->kp.sd) {
         throw Error(CCP_ERR_PARAM, "batch systems must share the kernel shape, timestep and temperature");
@@ -380,6 +383,7 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
     b->r.alloc(b->N);
     b->er.alloc(b->N);
     b->pstar.alloc(b->N);
+    if (b->kp.lanes == 8 && b->integrate_pass) b->fpair.alloc(b->N);
     b->dsys.alloc(n_sys);
     b->dslot.alloc(n_sys + 1);
     b->flags.alloc(FLAG_COUNT);

@@ -628,6 +628,42 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
   }
 }
 
+// Force cap and the stage update of bead i from its total force (the
+// epilogue of the force kernels): STAGE 0 writes the force, STAGE 1 the Heun
+// predictor and r + f dt/2 + noise, STAGE 2 the corrector; both bin the new
+// position.
+template <int STAGE>
+__device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q, uint32_t i, uint32_t id0, uint32_t w,
+                                               float4 me, float fx, float fy, float fz) {
+  if (STAGE != 0 && p.fcap2 > 0.0f) {
+    const float m2 = fx * fx + fy * fy + fz * fz;
+    if (m2 > p.fcap2) {
+      const float sc = sqrtf(p.fcap2 / m2);
+      fx *= sc; fy *= sc; fz *= sc;
+    }
+  }
+  if constexpr (STAGE == 0) {
+    q.out0[i] = make_float4(fx, fy, fz, 0.0f);
+  } else if constexpr (STAGE == 1) {
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (p.sd > 0.0f) {
+      float z0, z1, z2;
+      noise3(p.seed_lo, p.seed_hi, i - id0, *q.step, z0, z1, z2);
+      nx = p.sd * z0; ny = p.sd * z1; nz = p.sd * z2;
+    }
+    const float ex = me.x + fx * p.dt + nx, ey = me.y + fy * p.dt + ny, ez = me.z + fz * p.dt + nz;
+    q.out0[i] = make_float4(ex, ey, ez, me.w);
+    q.pstar[i] = make_float4(me.x + fx * p.half_dt + nx, me.y + fy * p.half_dt + ny, me.z + fz * p.half_dt + nz,
+                             0.0f);
+    bin_bead(p, q, i, w, ex, ey, ez);
+  } else {
+    const float4 ps = q.pstar[i];
+    const float nx = ps.x + fx * p.half_dt, ny = ps.y + fy * p.half_dt, nz = ps.z + fz * p.half_dt;
+    q.out0[i] = make_float4(nx, ny, nz, me.w);
+    bin_bead(p, q, i, w, nx, ny, nz);
+  }
+}
+
 // ---------------------------------------------------------------- balanced force kernel
 // Eight lanes per bead (LANES = 8). The bead's neighbour rows (2R+1 planes of
 // 2R+1 rows) are trimmed to the cells that can hold a bead within its own
@@ -677,7 +713,9 @@ __device__ __forceinline__ void trimmed_row(const GridGeom &g, int cx, int cy, i
   c1 = row + xh + 1;
 }
 
-template <int MODE, int STAGE, bool BATCH, int R>
+// EPI = false: the kernel stops after the pair sum and writes it to q.fpair;
+// k_integrate (one lane per bead) adds bonds and wall and runs the stage.
+template <int MODE, int STAGE, bool BATCH, int R, bool EPI = true>
 __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
   constexpr int W = 2 * R + 1, NROW = W * W, RPL = (NROW + C_LANES - 1) / C_LANES;
   __shared__ uint32_t s_list[C_BEADS_PER_BLOCK][C_LIST];
@@ -741,7 +779,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-  if (active && sub == 0) {
+  if (EPI && active && sub == 0) {
     bonded_force(p, q.pos, i, w, rx, ry, rz, fx, fy, fz);
     wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
   }
@@ -766,32 +804,37 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
     fz += __shfl_xor(fz, o);
   }
   if (!active || sub != 0) return;
-  if (STAGE != 0 && p.fcap2 > 0.0f) {
-    const float m2 = fx * fx + fy * fy + fz * fz;
-    if (m2 > p.fcap2) {
-      const float sc = sqrtf(p.fcap2 / m2);
-      fx *= sc; fy *= sc; fz *= sc;
-    }
-  }
-  if constexpr (STAGE == 0) {
-    q.out0[i] = make_float4(fx, fy, fz, 0.0f);
-  } else if constexpr (STAGE == 1) {
-    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    if (p.sd > 0.0f) {
-      float z0, z1, z2;
-      noise3(p.seed_lo, p.seed_hi, i - id0, *q.step, z0, z1, z2);
-      nx = p.sd * z0; ny = p.sd * z1; nz = p.sd * z2;
-    }
-    const float ex = rx + fx * p.dt + nx, ey = ry + fy * p.dt + ny, ez = rz + fz * p.dt + nz;
-    q.out0[i] = make_float4(ex, ey, ez, me.w);
-    q.pstar[i] = make_float4(rx + fx * p.half_dt + nx, ry + fy * p.half_dt + ny, rz + fz * p.half_dt + nz, 0.0f);
-    bin_bead(p, q, i, w, ex, ey, ez);
+  if constexpr (!EPI) {
+    q.fpair[i] = make_float4(fx, fy, fz, 0.0f);
   } else {
-    const float4 ps = q.pstar[i];
-    const float nx = ps.x + fx * p.half_dt, ny = ps.y + fy * p.half_dt, nz = ps.z + fz * p.half_dt;
-    q.out0[i] = make_float4(nx, ny, nz, me.w);
-    bin_bead(p, q, i, w, nx, ny, nz);
+    stage_epilogue<STAGE>(p, q, i, id0, w, me, fx, fy, fz);
   }
+}
+
+// Bonds, wall and the Heun stage of every bead, one lane per bead in index
+// order, on top of the pair forces k_force_bal<..., false> left in q.fpair.
+template <int MODE, int STAGE, bool BATCH>
+__global__ __launch_bounds__(BEAD_BLOCK) void k_integrate(KParams p, KPtrs q) {
+  const int t = xcd_block() * BEAD_BLOCK + threadIdx.x;
+  if (t >= p.N) return;
+  const uint32_t i = (uint32_t)t;
+  const float4 me = q.pos[i];
+  const uint32_t w = wbits(me);
+  uint32_t id0 = 0;
+  if constexpr (BATCH) {
+    const SysParams &S = q.sys[system_of(q.sys_slot, q.n_sys, i)];
+    p.R_n = S.R_n; p.R_o = S.R_o; p.R_b = S.R_b;
+    p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
+    p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
+    p.seed_lo = S.seed_lo; p.seed_hi = S.seed_hi;
+    p.gf = S.g;
+    id0 = S.slot;
+  }
+  const float4 fp = q.fpair[i];
+  float fx = fp.x, fy = fp.y, fz = fp.z;
+  bonded_force(p, q.pos, i, w, me.x, me.y, me.z, fx, fy, fz);
+  wall_force<MODE>(p, me.x, me.y, me.z, w, fx, fy, fz);
+  stage_epilogue<STAGE>(p, q, i, id0, w, me, fx, fy, fz);
 }
 
 // Binning of an index-ordered array (initial builds, LBS grid, export grid).
@@ -1148,6 +1191,15 @@ static void force_variant(hipStream_t s, int stage, const KParams &p, const KPtr
 template <int MODE, bool BATCH, int R>
 static void force_balanced(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
   const unsigned g = blocks_for(p.N, C_BEADS_PER_BLOCK);
+  if (q.fpair) {
+    // pair sum with eight lanes per bead, then bonds, wall and the stage with one
+    k_force_bal<MODE, 0, BATCH, R, false><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+    const unsigned gi = blocks_for(p.N, BEAD_BLOCK);
+    if (stage == 0) k_integrate<MODE, 0, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
+    else if (stage == 1) k_integrate<MODE, 1, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
+    else k_integrate<MODE, 2, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
+    return;
+  }
   if (stage == 0) k_force_bal<MODE, 0, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);
   else if (stage == 1) k_force_bal<MODE, 1, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);
   else k_force_bal<MODE, 2, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);

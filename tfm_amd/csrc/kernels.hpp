@@ -104,6 +104,7 @@ struct KPtrs {
   uint32_t *hcount, *hblocksum, *hbucket; // ... and into the next het grid
   uint32_t *flags;         // FLAG_COUNT error counters
   const uint64_t *step;    // global step counter (noise counter)
+  float4 *fpair;           // pair forces by bead index (balanced kernel with a separate integration pass)
   // batches only (sys == nullptr for a single system)
   const SysParams *sys;    // n_sys systems
   const uint32_t *sys_slot; // n_sys + 1 first slots

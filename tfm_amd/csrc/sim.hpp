@@ -110,6 +110,7 @@ struct ccp_sim {
   uint32_t graph_steps = 64;
   uint32_t reach = 1; // neighbour cells scanned on each side
   uint32_t lanes = 3; // lanes per bead of the force kernel
+  bool integrate_pass = false; // 8 lanes: bonds, wall and stage update in a one-lane-per-bead pass
   float fcap = 0.0f;
   KParams kp{};
 
@@ -125,7 +126,7 @@ struct ccp_sim {
   // device
   int dev = 0;
   hipStream_t stream = nullptr;
-  DevBuf<float4> r, er, pstar, lr, lspos, aux;
+  DevBuf<float4> r, er, pstar, lr, lspos, aux, fpair;
   DevBuf<uint32_t> lstart, flags, cmap;
   DevBuf<uint64_t> dstep;
   DevBuf<unsigned long long> dcount;
@@ -179,7 +180,7 @@ struct ccp_sim {
 
   ~ccp_sim() {
     drop_graphs();
-    DevBuf<float4> *f4[] = {&r, &er, &pstar, &lr, &lspos, &aux};
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &lr, &lspos, &aux, &fpair};
     for (auto *b : f4) b->free_();
     DevBuf<uint32_t> *u4[] = {&lstart, &flags, &cmap};
     for (auto *b : u4) b->free_();
