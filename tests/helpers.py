@@ -67,3 +67,9 @@ def pattern(N, seed, frac_het=0.5, mean=20):
         i += L
         het = not het
     return fl
+
+
+def lbs_sites(R_n, n, seed):
+    """n lamina binding sites uniformly on the sphere of radius R_n - rco."""
+    d = np.random.default_rng(seed).normal(size=(n, 3))
+    return (R_n - po.CCP_CONST["rco"]) * d / np.linalg.norm(d, axis=1, keepdims=True)

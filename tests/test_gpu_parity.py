@@ -4,14 +4,12 @@ import pytest
 
 import pyoracle as po
 from gpu_common import make_pair, step
-from helpers import pattern, radius_for, relaxed, walk
+from helpers import lbs_sites, pattern, radius_for, relaxed, walk
 
 pytestmark = pytest.mark.gpu
 
 
-def lbs_sites(R_n, n, seed):
-    d = np.random.default_rng(seed).normal(size=(n, 3))
-    return (R_n - po.CCP_CONST["rco"]) * d / np.linalg.norm(d, axis=1, keepdims=True)
+
 
 
 @pytest.mark.parametrize("N,cvf,seed", [(1, 0.2, 1), (1000, 0.2, 3), (16384, 0.2, 4), (131072, 0.3, 5)])
