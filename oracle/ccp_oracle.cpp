@@ -276,10 +276,21 @@ template <class R> inline V3<R> lbs_force(const Sys<R> &S, V3<R> vlp) {
 template <class R> struct Grid {
   std::vector<uint32_t> soi, beg, end;
 };
+// floor(v) as an int; values beyond +-2^30 and NaN (which only a blown-up
+// state produces) saturate, so the conversion is always defined and such
+// a bead lands outside every grid.
+template <class R> inline int floor_index(R v) {
+  const R lim = (R)(1 << 30);
+  if (!(v > -lim)) return v != v ? (1 << 30) : -(1 << 30);
+  if (v > lim) return 1 << 30;
+  return (int)std::floor(v);
+}
+// Cell key in ccp's mixed int/uint arithmetic (calc_indexes, sugrid.cu:29-34):
+// cps is unsigned there, so the sum wraps modulo 2^32.
 template <class R> inline uint32_t cell_key(R csl, uint32_t cps, V3<R> r) {
-  int ix = (int)std::floor(r.x / csl), iy = (int)std::floor(r.y / csl), iz = (int)std::floor(r.z / csl);
-  int iofst = (int)((cps / 2) * (1 + cps + cps * cps));
-  return (uint32_t)(iofst + ix + iy * (int)cps + iz * (int)(cps * cps));
+  const int ix = floor_index(r.x / csl), iy = floor_index(r.y / csl), iz = floor_index(r.z / csl);
+  const uint32_t iofst = (cps / 2) * (1 + cps + cps * cps);
+  return iofst + (uint32_t)ix + (uint32_t)iy * cps + (uint32_t)iz * cps * cps;
 }
 template <class R> int build_grid(R csl, uint32_t cps, const V3<R> *r, int n, Grid<R> &G, uint32_t *uci_out = nullptr,
                                   uint32_t *sci_out = nullptr) {
@@ -319,7 +330,7 @@ template <class R> int build_grid(R csl, uint32_t cps, const V3<R> *r, int n, Gr
 template <class R>
 V3<R> short_range_cells(const Sys<R> &S, const V3<R> *r, const Grid<R> &PG, const Grid<R> &LG, int i) {
   V3<R> ri = r[i];
-  int ix = (int)std::floor(ri.x / S.csl), iy = (int)std::floor(ri.y / S.csl), iz = (int)std::floor(ri.z / S.csl);
+  const int ix = floor_index(ri.x / S.csl), iy = floor_index(ri.y / S.csl), iz = floor_index(ri.z / S.csl);
   uint32_t cps = S.cps;
   uint32_t iofst = (cps / 2) * (1 + cps + cps * cps);
   V3<R> srf = {0, 0, 0};
