@@ -231,3 +231,26 @@ def test_full_size_workload_properties():
     m, _ = sim.contact_map()
     assert np.array_equal(m, po.contacts_grid(r, np.float32(1.5), 128, 1024))
     sim.close()
+
+
+def test_type_grouping_is_bit_exact():
+    # the het-first reordering of each workgroup's beads moves beads between
+    # wavefronts but keeps every bead's candidate list and summation order:
+    # trajectories of the benchmark workload must agree to the bit
+    step("type grouping bit-exact N=131072")
+    import bench
+
+    shape = {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.25}
+    a = tfm_amd.Simulation({**bench.PARAMS, **shape, "group_types": 1})
+    a.generate_initial_condition()
+    xyz, fl = a.positions, a.flags
+    b = tfm_amd.Simulation({**bench.PARAMS, **shape, "group_types": 0})
+    b.flags = fl
+    b.positions = xyz
+    for s in (a, b):
+        s.set_time(0, 0.0, 0)
+        s.run_steps(300)
+        s.synchronize()
+    assert np.array_equal(a.positions, b.positions)
+    a.close()
+    b.close()

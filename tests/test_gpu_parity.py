@@ -62,6 +62,8 @@ FORCE_CASES = [
     # eight lanes for the pairs, bonds / wall / stage in a second one-lane-per-bead pass
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, integrate_pass=1),
     dict(ff=po.FF_CCP, n_l=40, lanes_per_bead=8, integrate_pass=1),
+    # eight lanes without the het-first reordering of each workgroup's beads
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, group_types=0),
     # pair sum split by cutoff over a fine grid and a het grid
     dict(ff=po.FF_CCP, n_l=40, fine_cell_size=1.2),
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, fine_cell_size=1.25),

@@ -139,6 +139,9 @@ void ccp_sim::configure(const ParMap &par) {
   // with 8 lanes per bead: bonds, wall and the stage in a second, one-lane-per-bead
   // pass (off by default: measured 15 % slower at N = 131072)
   integrate_pass = par.get<uint32_t>("integrate_pass", 0u) != 0;
+  // with 8 lanes per bead: reorder each workgroup's beads by cutoff so that a
+  // wavefront's lists have similar lengths (no change to any result)
+  group_types = par.get<uint32_t>("group_types", 1u) != 0;
   graph_steps = par.get<uint32_t>("graph_steps", 64u);
   if (!(1 <= graph_steps && graph_steps <= 4096)) throw Error(CCP_ERR_PARAM, "graph_steps out of range");
 
@@ -202,6 +205,7 @@ void ccp_sim::fill_kparams() {
   if (!split) k.gh.n_c = 0;
   k.split = split ? 1 : 0;
   k.lanes = (int)lanes;
+  k.group = group_types ? 1 : 0;
   k.reach = split ? 1 : (int)reach;
   k.R_n = R_n; k.R_o = R_o; k.R_b = R_b;
   k.noc = noc; k.boc = boc; k.nod = nod; k.bod = bod;
@@ -228,7 +232,12 @@ void ccp_sim::fill_kparams() {
   k.eps_wall = 1.0f;
   k.eps_lam = eps_lam;
   k.rc_lam2 = 2.5f * 2.5f;
-  (void)lj;
+  {
+    const double margin = 1e-3; // covers the rounding of cell edges
+    const double rl = lj ? 2.5 : ACO, rs = lj ? std::pow(2.0, 1.0 / 6.0) : RCO;
+    k.trim2_long = (float)((rl + margin) * (rl + margin));
+    k.trim2_short = (float)((rs + margin) * (rs + margin));
+  }
   k.dt = dt;
   k.half_dt = 0.5f * dt;
   k.sd = sd;

@@ -674,43 +674,68 @@ __device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q,
 constexpr int C_LANES = 8;
 constexpr int C_BEADS_PER_BLOCK = (64 / C_LANES) * (BEAD_BLOCK / 64);
 constexpr int C_LIST = 128; // candidate slots per bead held in LDS
+static_assert(C_BEADS_PER_BLOCK <= 32, "the type grouping keeps one bit per bead in 32 bits");
 
-// Cutoff of bead w's pairs, plus a margin for the rounding of cell edges.
-template <int MODE> __device__ __forceinline__ float trim_radius(const KParams &p, uint32_t w) {
-  float rc;
-  if constexpr (MODE == MODE_CCP) rc = (w & W_HET) ? sqrtf(p.aco2) : p.rco;
-  else if constexpr (MODE == MODE_ICG) rc = p.rco;
-  else rc = (w & W_HET) ? sqrtf(p.rc_lj2) : sqrtf(p.rm2);
-  return rc + 1e-3f;
+// Squared list radius of bead w: its cutoff (the het-het one for het beads,
+// the short one otherwise) plus 1e-3 sigma for the rounding of cell edges.
+template <int MODE> __device__ __forceinline__ float trim_radius2(const KParams &p, uint32_t w) {
+  if constexpr (MODE == MODE_ICG) {
+    const float r = p.rco + 1e-3f;
+    return r * r;
+  }
+  return (w & W_HET) ? p.trim2_long : p.trim2_short;
 }
 
-// Cells [c0, c1) of row (cy + dy, cz + dz) that can hold a bead within rc of
-// (x, y, z): empty when the row's y-z slab is farther than rc, else the
-// cells met by the chord of half-length sqrt(rc^2 - gap^2), within reach R.
+// Squared gaps along one axis between a bead at offset l from its cell's low
+// edge and the cells k + 1 steps below (lo[k]) and above (hi[k]) its own.
+template <int R> struct AxisGaps {
+  float lo[R], hi[R];
+};
+template <int R> __device__ __forceinline__ AxisGaps<R> axis_gaps(float l, float csl) {
+  AxisGaps<R> G;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const float a = fmaxf(l + (float)k * csl, 0.0f), b = fmaxf((float)(k + 1) * csl - l, 0.0f);
+    G.lo[k] = a * a;
+    G.hi[k] = b * b;
+  }
+  return G;
+}
+// squared gap to the cell d in [-R, R] steps away (0 for the bead's own)
+template <int R> __device__ __forceinline__ float gap2(const AxisGaps<R> &G, int d) {
+  float v = 0.0f;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    v = d == -(k + 1) ? G.lo[k] : v;
+    v = d == k + 1 ? G.hi[k] : v;
+  }
+  return v;
+}
+
+// Cells [c0, c1) of row (cy + dy, cz + dz) that can hold a bead within
+// sqrt(rc2) of the bead: empty when the row's y-z slab is farther, else the
+// cells of the row, within reach R, whose x gap is at most the remaining
+// sqrt(rc2 - gap_y^2 - gap_z^2). The gaps grow with the distance in cells,
+// so the kept cells are a contiguous run around cx.
 template <int R>
-__device__ __forceinline__ void trimmed_row(const GridGeom &g, int cx, int cy, int cz, float x, float y, float z,
-                                            int dy, int dz, float rc, int &c0, int &c1) {
-  c0 = c1 = 0;
+__device__ __forceinline__ void trimmed_row(const GridGeom &g, int cx, int cy, int cz, const AxisGaps<R> &X,
+                                            const AxisGaps<R> &Y, const AxisGaps<R> &Z, int dy, int dz, float rc2,
+                                            bool want, int &c0, int &c1) {
+  // branch-free: every lane evaluates the row and an empty one selects 0
   const int yy = cy + dy, zz = cz + dz;
-  if (yy < 0 || yy >= g.cps || zz < 0 || zz >= g.cps) return;
-  float gy = 0.0f, gz = 0.0f;
-  if (dy < 0) gy = y - (float)(yy - g.half + 1) * g.csl;
-  else if (dy > 0) gy = (float)(yy - g.half) * g.csl - y;
-  if (dz < 0) gz = z - (float)(zz - g.half + 1) * g.csl;
-  else if (dz > 0) gz = (float)(zz - g.half) * g.csl - z;
-  gy = fmaxf(gy, 0.0f);
-  gz = fmaxf(gz, 0.0f);
-  const float h2 = rc * rc - gy * gy - gz * gz;
-  if (h2 < 0.0f) return;
-  // a reciprocal is close enough here: the margin in rc keeps the range
-  // conservative by far more than its rounding
-  const float h = sqrtf(h2), ic = __builtin_amdgcn_rcpf(g.csl);
-  const int xl = max((int)floorf((x - h) * ic) + g.half, max(cx - R, 0));
-  const int xh = min((int)floorf((x + h) * ic) + g.half, min(cx + R, g.cps - 1));
-  if (xl > xh) return;
+  const float h2 = rc2 - gap2<R>(Y, dy) - gap2<R>(Z, dz);
+  const bool ok = want && (unsigned)yy < (unsigned)g.cps && (unsigned)zz < (unsigned)g.cps && h2 >= 0.0f;
+  int xl = cx, xh = cx;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    xl -= X.lo[k] <= h2 ? 1 : 0;
+    xh += X.hi[k] <= h2 ? 1 : 0;
+  }
+  xl = max(xl, 0);
+  xh = min(xh, g.cps - 1);
   const int row = g.coff + (zz * g.cps + yy) * g.cps;
-  c0 = row + xl;
-  c1 = row + xh + 1;
+  c0 = ok ? row + xl : 0;
+  c1 = ok ? row + xh + 1 : 0;
 }
 
 // EPI = false: the kernel stops after the pair sum and writes it to q.fpair;
@@ -721,7 +746,38 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
   __shared__ uint32_t s_list[C_BEADS_PER_BLOCK][C_LIST];
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const int sub = lane % C_LANES, grp = wave * (64 / C_LANES) + lane / C_LANES;
-  const int t = xcd_block() * C_BEADS_PER_BLOCK + grp;
+  const int t0 = xcd_block() * C_BEADS_PER_BLOCK;
+  int t = t0 + grp;
+  if (p.group) {
+    // A wavefront runs as long as its longest list. Het beads (cutoff 2.5
+    // sigma) have about four times the candidates of the others, so the
+    // workgroup's slots are reordered: het slots first, then the rest, each
+    // in cell order. Position grp takes the k-th set bit of the het mask.
+    __shared__ uint32_t s_mask[BEAD_BLOCK / 64];
+    const bool het = t < p.N && (wbits(q.fspos[t]) & W_HET) != 0u;
+    const uint64_t b = __ballot(het && sub == 0);
+    uint32_t m8 = 0;
+#pragma unroll
+    for (int k = 0; k < 64 / C_LANES; ++k) m8 |= (uint32_t)((b >> (k * C_LANES)) & 1u) << k;
+    if (lane == 0) s_mask[wave] = m8;
+    __syncthreads();
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < BEAD_BLOCK / 64; ++k) m |= s_mask[k] << (k * (64 / C_LANES));
+    const int nh = __popc(m);
+    uint32_t mm = grp < nh ? m : ~m;
+    int k = grp < nh ? grp : grp - nh, pos = 0;
+#pragma unroll
+    for (int sh = C_BEADS_PER_BLOCK / 2; sh > 0; sh >>= 1) {
+      const int c = __popc(mm & ((1u << sh) - 1u));
+      if (k >= c) {
+        k -= c;
+        mm >>= sh;
+        pos += sh;
+      }
+    }
+    t = t0 + pos;
+  }
   const bool active = t < p.N;
   const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const uint32_t w = wbits(me);
@@ -740,7 +796,10 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
   bool outside = false;
   const int cx = cell_coord(rx, p.gf, outside), cy = cell_coord(ry, p.gf, outside),
             cz = cell_coord(rz, p.gf, outside);
-  const float rc = trim_radius<MODE>(p, w);
+  const float rc2 = trim_radius2<MODE>(p, w);
+  const AxisGaps<R> GX = axis_gaps<R>(rx - (float)(cx - p.gf.half) * p.gf.csl, p.gf.csl),
+                    GY = axis_gaps<R>(ry - (float)(cy - p.gf.half) * p.gf.csl, p.gf.csl),
+                    GZ = axis_gaps<R>(rz - (float)(cz - p.gf.half) * p.gf.csl, p.gf.csl);
 
   // rows sub, sub + 8, ...: trimmed slot ranges and the LBS sites of LAD beads
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -748,8 +807,8 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
 #pragma unroll
   for (int m = 0; m < RPL; ++m) {
     const int r = sub + m * C_LANES;
-    int c0 = 0, c1 = 0;
-    if (active && r < NROW) trimmed_row<R>(p.gf, cx, cy, cz, rx, ry, rz, r % W - R, r / W - R, rc, c0, c1);
+    int c0, c1;
+    trimmed_row<R>(p.gf, cx, cy, cz, GX, GY, GZ, r % W - R, r / W - R, rc2, active && r < NROW, c0, c1);
     lo[m] = q.fcstart[c0];
     len[m] = q.fcstart[c1] - lo[m];
     tot += len[m];

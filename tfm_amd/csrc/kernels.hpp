@@ -69,6 +69,7 @@ struct KParams {
   int split;   // 1: short-range pairs on gf + het-het tail on gh; 0: every pair on gf
   int reach;   // neighbour cells scanned on each side: 1 (27 cells) or 2 (125 cells of half the side)
   int lanes;   // lanes per bead of the force kernel: 2 reach + 1 (one per z-plane), 1, or 8 (balanced list)
+  int group;   // 8 lanes: within a workgroup, beads with the long (het) cutoff take the first wavefronts
   // nucleus geometry (cngeom, chrdat.cuh:48-69)
   float R_n, R_o, R_b, noc, boc, nod, bod;
   // bonds and bending
@@ -81,6 +82,10 @@ struct KParams {
   float lfac, lco2, inv_lco2;
   // WCA/LJ set
   float eps_wca, sig2, rm2, eps_het, rc_lj2, eps_wall, eps_lam, rc_lam2;
+  // squared candidate-list radii of the balanced kernel: cutoff + 1e-3 sigma
+  // for het beads (the het-het cutoff) and for the others (the short one), in
+  // the simulation's pair set (the relaxation set uses rco for every bead)
+  float trim2_long, trim2_short;
   // integrator
   float dt, half_dt, sd;
   float fcap2; // squared force cap of the integrator (0 = off)

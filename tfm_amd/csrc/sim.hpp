@@ -111,6 +111,7 @@ struct ccp_sim {
   uint32_t reach = 1; // neighbour cells scanned on each side
   uint32_t lanes = 3; // lanes per bead of the force kernel
   bool integrate_pass = false; // 8 lanes: bonds, wall and stage update in a one-lane-per-bead pass
+  bool group_types = true;     // 8 lanes: het beads first within each workgroup
   float fcap = 0.0f;
   KParams kp{};
 
