@@ -64,6 +64,14 @@ FORCE_CASES = [
     dict(ff=po.FF_CCP, n_l=40, lanes_per_bead=8, integrate_pass=1),
     # eight lanes without the het-first reordering of each workgroup's beads
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, group_types=0),
+    # eight lanes per bead, the pair phase of 2, 4 or 8 groups of eight beads
+    # per wavefront before the per-bead work
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, bal_rounds=2),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, bal_rounds=4),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, bal_rounds=8),
+    dict(ff=po.FF_CCP, n_l=40, lanes_per_bead=8, bal_rounds=4),
+    dict(ff=po.FF_ICG, n_l=0, lanes_per_bead=8, bal_rounds=8),
+    dict(ff=po.FF_CCP, n_l=20, opening_radius=4.0, bleb_radius=6.0, cell_reach=2, lanes_per_bead=8, bal_rounds=4),
     # pair sum split by cutoff over a fine grid and a het grid
     dict(ff=po.FF_CCP, n_l=40, fine_cell_size=1.2),
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, fine_cell_size=1.25),
@@ -100,11 +108,12 @@ def test_forces_match_oracle(case):
     sim.close()
 
 
-@pytest.mark.parametrize("ff,dt,tol,reach,lanes", [(po.FF_CCP, 1 / 2048, 1e-3, 1, 3), (po.FF_LJ, 1e-3, 2e-3, 1, 3),
-                                                   (po.FF_LJ, 1e-3, 2e-3, 2, 5), (po.FF_CCP, 1 / 2048, 1e-3, 1, 8),
-                                                   (po.FF_LJ, 1e-3, 2e-3, 2, 8)])
-def test_trajectory_1000_steps(ff, dt, tol, reach, lanes):
-    step(f"trajectory ff={ff} reach={reach} lanes={lanes}")
+@pytest.mark.parametrize("ff,dt,tol,reach,lanes,rounds", [
+    (po.FF_CCP, 1 / 2048, 1e-3, 1, 3, 1), (po.FF_LJ, 1e-3, 2e-3, 1, 3, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 5, 1),
+    (po.FF_CCP, 1 / 2048, 1e-3, 1, 8, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 8, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 8, 4),
+    (po.FF_CCP, 1 / 2048, 1e-3, 1, 8, 8)])
+def test_trajectory_1000_steps(ff, dt, tol, reach, lanes, rounds):
+    step(f"trajectory ff={ff} reach={reach} lanes={lanes} rounds={rounds}")
     N = 1024
     R_n = radius_for(N, 0.2)
     r = relaxed(N, R_n, 2)
@@ -115,7 +124,8 @@ def test_trajectory_1000_steps(ff, dt, tol, reach, lanes):
     else:
         fl = pattern(N, 1)
         lbs = lbs_sites(R_n, 30, 8)
-    sim, osys, r64 = make_pair(N, R_n, fl, r, ff=ff, dt=dt, lbs=lbs, cell_reach=reach, lanes_per_bead=lanes)
+    sim, osys, r64 = make_pair(N, R_n, fl, r, ff=ff, dt=dt, lbs=lbs, cell_reach=reach, lanes_per_bead=lanes,
+                               **({"bal_rounds": rounds} if rounds > 1 else {}))
     sim.set_time(0, 0.0, 0)
     sim.run_steps(1000)
     sim.synchronize()

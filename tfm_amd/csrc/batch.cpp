@@ -69,12 +69,13 @@ struct ccp_batch {
   std::vector<uint32_t> hbond;       // W_B* bits by bead
   uint64_t step = 0;
   bool dstep_dirty = true;
+  uint64_t noise_step = ~0ull; // step whose normals znoise holds
   uint32_t n_samples = 0;
   std::vector<double> rd_sum;        // n_sys x rd_bins
   // device
   int dev = 0;
   hipStream_t stream = nullptr;
-  DevBuf<float4> r, er, pstar, fpair;
+  DevBuf<float4> r, er, pstar, fpair, znoise;
   DevBuf<SysParams> dsys;
   DevBuf<uint32_t> dslot, flags, cmaps;
   DevBuf<uint64_t> dstep;
@@ -92,7 +93,7 @@ struct ccp_batch {
   }
   ~ccp_batch() {
     drop_graphs();
-    DevBuf<float4> *f4[] = {&r, &er, &pstar, &fpair};
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &fpair, &znoise};
     for (auto *b : f4) b->free_();
     DevBuf<uint32_t> *u4[] = {&dslot, &flags, &cmaps};
     for (auto *b : u4) b->free_();
@@ -177,6 +178,16 @@ struct ccp_batch {
       dstep_dirty = false;
     }
   }
+  // normals of every system's beads, each under its own key and local index
+  NoiseJob noise_job(uint32_t ahead) const {
+    return NoiseJob{znoise.p, dstep.p, ahead, N, 0u, 0u, dsys.p, dslot.p, (int)n_sys};
+  }
+  void ensure_noise() {
+    sync_step();
+    if (noise_step == step) return;
+    launch_noise_fill(stream, noise_job(0));
+    noise_step = step;
+  }
   void launch_stage(int stage) {
     KPtrs q{};
     q.fspos = grid.spos.p;
@@ -190,13 +201,15 @@ struct ccp_batch {
     q.flags = flags.p;
     q.step = dstep.p;
     q.fpair = fpair.p;
+    q.noise = znoise.p;
     q.sys = dsys.p;
     q.sys_slot = dslot.p;
     q.n_sys = (int)n_sys;
     launch_force(stream, MODE_LJ, stage, kp, q);
     OrderJob j{grid.g, grid.count.p, grid.bs(grid.clean), grid.bs(grid.clean ^ 1), grid.bucket.p, q.out0,
                grid.spos.p, nullptr, grid.cstart.p};
-    launch_order(stream, j, nullptr, flags.p, stage == 2 ? dstep.p : nullptr);
+    const NoiseJob nj = noise_job(1);
+    launch_order(stream, j, nullptr, flags.p, stage == 2 ? dstep.p : nullptr, stage == 1 ? &nj : nullptr);
     grid.clean ^= 1;
   }
   void run_chunk(uint64_t n) {
@@ -254,12 +267,13 @@ struct ccp_batch {
   }
   void run_steps(uint64_t n) {
     ensure_grid();
-    sync_step();
+    ensure_noise();
     while (n > 0) {
       uint64_t chunk = n;
       if (cinterval) chunk = std::min<uint64_t>(n, cinterval - step % cinterval);
       run_chunk(chunk);
       step += chunk;
+      noise_step = step;
       n -= chunk;
       if (cinterval && step % cinterval == 0) sample();
     }
@@ -383,6 +397,7 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
     b->r.alloc(b->N);
     b->er.alloc(b->N);
     b->pstar.alloc(b->N);
+    b->znoise.alloc(b->N);
     if (b->kp.lanes == 8 && b->integrate_pass) b->fpair.alloc(b->N);
     b->dsys.alloc(n_sys);
     b->dslot.alloc(n_sys + 1);
@@ -519,7 +534,7 @@ int ccp_batch_time_steps(ccp_batch *b, uint32_t n, float *ms_per_step) {
   return bguarded(b, [&] {
     bneed(ms_per_step && n > 0, "bad arguments");
     b->ensure_grid();
-    b->sync_step();
+    b->ensure_noise();
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0));
     hip_check(hipEventCreate(&e1));
@@ -536,6 +551,7 @@ int ccp_batch_time_steps(ccp_batch *b, uint32_t n, float *ms_per_step) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->step += n;
+    b->noise_step = b->step;
     b->check_flags();
   });
 }

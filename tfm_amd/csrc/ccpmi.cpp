@@ -142,6 +142,11 @@ void ccp_sim::configure(const ParMap &par) {
   // with 8 lanes per bead: reorder each workgroup's beads by cutoff so that a
   // wavefront's lists have similar lengths (no change to any result)
   group_types = par.get<uint32_t>("group_types", 1u) != 0;
+  // with 8 lanes per bead: a wavefront runs the pair phase for `bal_rounds`
+  // groups of eight beads, then the per-bead work of all of them at once
+  rounds = par.get<uint32_t>("bal_rounds", 1u);
+  if (rounds != 1 && rounds != 2 && rounds != 4 && rounds != 8) throw Error(CCP_ERR_PARAM, "bal_rounds out of range");
+  if (rounds > 1 && integrate_pass) throw Error(CCP_ERR_PARAM, "bal_rounds > 1 excludes integrate_pass");
   graph_steps = par.get<uint32_t>("graph_steps", 64u);
   if (!(1 <= graph_steps && graph_steps <= 4096)) throw Error(CCP_ERR_PARAM, "graph_steps out of range");
 
@@ -206,6 +211,7 @@ void ccp_sim::fill_kparams() {
   k.split = split ? 1 : 0;
   k.lanes = (int)lanes;
   k.group = group_types ? 1 : 0;
+  k.rounds = (int)rounds;
   k.reach = split ? 1 : (int)reach;
   k.R_n = R_n; k.R_o = R_o; k.R_b = R_b;
   k.noc = noc; k.boc = boc; k.nod = nod; k.bod = bod;
@@ -254,7 +260,7 @@ void ccp_sim::fill_kparams() {
 void ccp_sim::allocate() {
   hip_check(hipGetDevice(&dev));
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  r.alloc(N); er.alloc(N); pstar.alloc(N); aux.alloc(N);
+  r.alloc(N); er.alloc(N); pstar.alloc(N); aux.alloc(N); znoise.alloc(N);
   if (lanes == 8 && integrate_pass) fpair.alloc(N);
   lr.alloc(n_l); lspos.alloc(n_l); lstart.alloc(gf.g.n_c + 1);
   flags.alloc(FLAG_COUNT);
@@ -396,7 +402,7 @@ void ccp_sim::sync_step() {
   }
 }
 
-void ccp_sim::launch_stage(int mode, int stage) {
+void ccp_sim::launch_stage(int mode, int stage, hipEvent_t mid) {
   KParams k = mode_params(mode);
   const bool het = split && mode != MODE_ICG;
   if (!het) k.gh.n_c = 0; // no het grid (single grid, or the soft relaxation)
@@ -419,12 +425,16 @@ void ccp_sim::launch_stage(int mode, int stage) {
   q.flags = flags.p;
   q.step = dstep.p;
   q.fpair = fpair.p;
+  q.noise = znoise.p;
   launch_force(stream, mode, stage, k, q);
+  if (mid) hip_check(hipEventRecord(mid, stream));
   OrderJob jf{gf.g, gf.count.p, gf.bs(gf.clean), gf.bs(gf.clean ^ 1), gf.bucket.p, q.out0, gf.spos.p, nullptr,
               gf.cstart.p};
   OrderJob jh{gh.g, gh.count.p, gh.bs(gh.clean), gh.bs(gh.clean ^ 1), gh.bucket.p, q.out0, gh.spos.p, nullptr,
               gh.cstart.p};
-  launch_order(stream, jf, het ? &jh : nullptr, flags.p, stage == 2 ? dstep.p : nullptr);
+  // after the first stage the counter still holds this step: draw the next one's
+  const NoiseJob nj = noise_job(1);
+  launch_order(stream, jf, het ? &jh : nullptr, flags.p, stage == 2 ? dstep.p : nullptr, stage == 1 ? &nj : nullptr);
   gf.clean ^= 1;
   if (het) gh.clean ^= 1;
   else gh.valid = false;
@@ -462,12 +472,13 @@ void ccp_sim::run_chunk(uint64_t n, int mode) {
 
 void ccp_sim::run_steps(uint64_t n, int mode) {
   ensure_grids();
-  sync_step();
+  ensure_noise();
   while (n > 0) {
     uint64_t chunk = n;
     if (cinterval && mode != MODE_ICG) chunk = std::min<uint64_t>(n, cinterval - step % cinterval);
     run_chunk(chunk, mode);
     step += chunk;
+    noise_step = step;
     n -= chunk;
     if (cinterval && mode != MODE_ICG && step % cinterval == 0) {
       launch_contacts(stream, kp, gf.spos.p, gf.cstart.p, cmap.p);
@@ -1277,44 +1288,16 @@ int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]) {
     need(out && n_steps > 0, "bad timing request");
     const int mode = h->ff == CCP_FF_LJ ? MODE_LJ : MODE_CCP;
     h->ensure_grids();
-    h->sync_step();
+    h->ensure_noise();
     hipEvent_t ev[5];
     for (auto &e : ev) hip_check(hipEventCreate(&e));
     double acc[4] = {0, 0, 0, 0};
     for (uint32_t s = 0; s < n_steps; ++s) {
       for (int stage = 1; stage <= 2; ++stage) {
-        // launch_stage = force launch + order launch; bracket each with events
-        KParams k = h->mode_params(mode);
-        KPtrs q{};
-        q.fspos = h->gf.spos.p;
-        q.fcstart = h->gf.cstart.p;
-        q.hspos = h->gh.spos.p;
-        q.hcstart = h->gh.cstart.p;
-        q.lspos = h->lspos.p;
-        q.lstart = h->lstart.p;
-        q.pos = stage == 1 ? h->r.p : h->er.p;
-        q.out0 = stage == 1 ? h->er.p : h->r.p;
-        q.pstar = h->pstar.p;
-        q.fcount = h->gf.count.p;
-        q.fblocksum = h->gf.bs(h->gf.clean);
-        q.fbucket = h->gf.bucket.p;
-        q.hcount = h->gh.count.p;
-        q.hblocksum = h->gh.bs(h->gh.clean);
-        q.hbucket = h->gh.bucket.p;
-        q.flags = h->flags.p;
-        q.step = h->dstep.p;
-        q.fpair = h->fpair.p;
+        // force launch | order launch, bracketed by events
         hip_check(hipEventRecord(ev[0], h->stream));
-        launch_force(h->stream, mode, stage, k, q);
-        hip_check(hipEventRecord(ev[1], h->stream));
-        OrderJob jf{h->gf.g, h->gf.count.p, h->gf.bs(h->gf.clean), h->gf.bs(h->gf.clean ^ 1), h->gf.bucket.p,
-                    q.out0, h->gf.spos.p, nullptr, h->gf.cstart.p};
-        OrderJob jh{h->gh.g, h->gh.count.p, h->gh.bs(h->gh.clean), h->gh.bs(h->gh.clean ^ 1), h->gh.bucket.p,
-                    q.out0, h->gh.spos.p, nullptr, h->gh.cstart.p};
-        launch_order(h->stream, jf, h->split ? &jh : nullptr, h->flags.p, stage == 2 ? h->dstep.p : nullptr);
+        h->launch_stage(mode, stage, ev[1]);
         hip_check(hipEventRecord(ev[2], h->stream));
-        h->gf.clean ^= 1;
-        if (h->split) h->gh.clean ^= 1;
         hip_check(hipEventSynchronize(ev[2]));
         float a, b;
         hip_check(hipEventElapsedTime(&a, ev[0], ev[1]));
@@ -1324,6 +1307,7 @@ int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]) {
       }
     }
     h->step += n_steps;
+    h->noise_step = h->step;
     // whole steps back to back (no events between kernels)
     hip_check(hipEventRecord(ev[3], h->stream));
     for (uint32_t s = 0; s < n_steps; ++s) h->launch_step(mode);
@@ -1332,6 +1316,7 @@ int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]) {
     float whole;
     hip_check(hipEventElapsedTime(&whole, ev[3], ev[4]));
     h->step += n_steps;
+    h->noise_step = h->step;
     for (int k = 0; k < 4; ++k) out[k] = (float)(acc[k] / n_steps);
     out[4] = whole / n_steps;
     for (auto &e : ev) (void)hipEventDestroy(e);

@@ -519,6 +519,29 @@ __device__ __forceinline__ int system_of(const uint32_t *__restrict__ sys_slot, 
   return lo;
 }
 
+// Constants of the batch system holding slot t.
+__device__ __forceinline__ void load_system(KParams &p, const KPtrs &q, uint32_t t) {
+  const SysParams &S = q.sys[system_of(q.sys_slot, q.n_sys, t)];
+  p.R_n = S.R_n; p.R_o = S.R_o; p.R_b = S.R_b;
+  p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
+  p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
+  p.gf = S.g;
+}
+
+// Unit normal triple of bead g (index over all systems) for the job's step.
+__device__ __forceinline__ void fill_noise(const NoiseJob &nj, uint32_t g) {
+  uint32_t k0 = nj.seed_lo, k1 = nj.seed_hi, local = g;
+  if (nj.sys) {
+    const SysParams &S = nj.sys[system_of(nj.sys_slot, nj.n_sys, g)];
+    k0 = S.seed_lo;
+    k1 = S.seed_hi;
+    local = g - S.slot;
+  }
+  float z0, z1, z2;
+  const uint32_t a = noise3(k0, k1, local, *nj.step + nj.ahead, z0, z1, z2);
+  nj.z[g] = make_float4(z0, z1, z2, __uint_as_float(a));
+}
+
 // Force evaluation of one stage, beads in grid order.
 // STAGE 0: write F(pos) by bead index. STAGE 1: Heun predictor
 // (exec_RK_1, chrsim.cu:364-388) er = r + f dt + rn, keeps
@@ -540,16 +563,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
   const uint32_t w = wbits(me);
   const uint32_t i = w & ID_MASK;
   const float rx = me.x, ry = me.y, rz = me.z;
-  uint32_t id0 = 0;
-  if constexpr (BATCH) {
-    const SysParams &S = q.sys[system_of(q.sys_slot, q.n_sys, active ? (uint32_t)t : 0u)];
-    p.R_n = S.R_n; p.R_o = S.R_o; p.R_b = S.R_b;
-    p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
-    p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
-    p.seed_lo = S.seed_lo; p.seed_hi = S.seed_hi;
-    p.gf = S.g;
-    id0 = S.slot;
-  }
+  if constexpr (BATCH) load_system(p, q, active ? (uint32_t)t : 0u);
 
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   if (active && part == 0) {
@@ -612,9 +626,8 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
   } else if constexpr (STAGE == 1) {
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
     if (p.sd > 0.0f) {
-      float z0, z1, z2;
-      noise3(p.seed_lo, p.seed_hi, i - id0, *q.step, z0, z1, z2);
-      nx = p.sd * z0; ny = p.sd * z1; nz = p.sd * z2;
+      const float4 z = q.noise[i];
+      nx = p.sd * z.x; ny = p.sd * z.y; nz = p.sd * z.z;
     }
     const float ex = rx + fx * p.dt + nx, ey = ry + fy * p.dt + ny, ez = rz + fz * p.dt + nz;
     q.out0[i] = make_float4(ex, ey, ez, me.w);
@@ -633,7 +646,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
 // predictor and r + f dt/2 + noise, STAGE 2 the corrector; both bin the new
 // position.
 template <int STAGE>
-__device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q, uint32_t i, uint32_t id0, uint32_t w,
+__device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q, uint32_t i, uint32_t w,
                                                float4 me, float fx, float fy, float fz) {
   if (STAGE != 0 && p.fcap2 > 0.0f) {
     const float m2 = fx * fx + fy * fy + fz * fz;
@@ -647,9 +660,8 @@ __device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q,
   } else if constexpr (STAGE == 1) {
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
     if (p.sd > 0.0f) {
-      float z0, z1, z2;
-      noise3(p.seed_lo, p.seed_hi, i - id0, *q.step, z0, z1, z2);
-      nx = p.sd * z0; ny = p.sd * z1; nz = p.sd * z2;
+      const float4 z = q.noise[i];
+      nx = p.sd * z.x; ny = p.sd * z.y; nz = p.sd * z.z;
     }
     const float ex = me.x + fx * p.dt + nx, ey = me.y + fy * p.dt + ny, ez = me.z + fz * p.dt + nz;
     q.out0[i] = make_float4(ex, ey, ez, me.w);
@@ -738,61 +750,17 @@ __device__ __forceinline__ void trimmed_row(const GridGeom &g, int cx, int cy, i
   c1 = ok ? row + xh + 1 : 0;
 }
 
-// EPI = false: the kernel stops after the pair sum and writes it to q.fpair;
-// k_integrate (one lane per bead) adds bonds and wall and runs the stage.
-template <int MODE, int STAGE, bool BATCH, int R, bool EPI = true>
-__global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
+
+// Pair (and LBS) forces of the bead `me` over its trimmed candidate list,
+// spread over the bead's eight lanes (sub = 0..7) and added to f; on return
+// every lane of the group holds the bead's total. `list` is the group's LDS
+// list; the caller's lanes of one wavefront run this together.
+template <int MODE, bool BATCH, int R>
+__device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, uint32_t *list, bool active,
+                                             int sub, int lane, float4 me, float &fx, float &fy, float &fz) {
   constexpr int W = 2 * R + 1, NROW = W * W, RPL = (NROW + C_LANES - 1) / C_LANES;
-  __shared__ uint32_t s_list[C_BEADS_PER_BLOCK][C_LIST];
-  const int lane = __lane_id(), wave = threadIdx.x >> 6;
-  const int sub = lane % C_LANES, grp = wave * (64 / C_LANES) + lane / C_LANES;
-  const int t0 = xcd_block() * C_BEADS_PER_BLOCK;
-  int t = t0 + grp;
-  if (p.group) {
-    // A wavefront runs as long as its longest list. Het beads (cutoff 2.5
-    // sigma) have about four times the candidates of the others, so the
-    // workgroup's slots are reordered: het slots first, then the rest, each
-    // in cell order. Position grp takes the k-th set bit of the het mask.
-    __shared__ uint32_t s_mask[BEAD_BLOCK / 64];
-    const bool het = t < p.N && (wbits(q.fspos[t]) & W_HET) != 0u;
-    const uint64_t b = __ballot(het && sub == 0);
-    uint32_t m8 = 0;
-#pragma unroll
-    for (int k = 0; k < 64 / C_LANES; ++k) m8 |= (uint32_t)((b >> (k * C_LANES)) & 1u) << k;
-    if (lane == 0) s_mask[wave] = m8;
-    __syncthreads();
-    uint32_t m = 0;
-#pragma unroll
-    for (int k = 0; k < BEAD_BLOCK / 64; ++k) m |= s_mask[k] << (k * (64 / C_LANES));
-    const int nh = __popc(m);
-    uint32_t mm = grp < nh ? m : ~m;
-    int k = grp < nh ? grp : grp - nh, pos = 0;
-#pragma unroll
-    for (int sh = C_BEADS_PER_BLOCK / 2; sh > 0; sh >>= 1) {
-      const int c = __popc(mm & ((1u << sh) - 1u));
-      if (k >= c) {
-        k -= c;
-        mm >>= sh;
-        pos += sh;
-      }
-    }
-    t = t0 + pos;
-  }
-  const bool active = t < p.N;
-  const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const uint32_t w = wbits(me);
-  const uint32_t i = w & ID_MASK;
   const float rx = me.x, ry = me.y, rz = me.z;
-  uint32_t id0 = 0;
-  if constexpr (BATCH) {
-    const SysParams &S = q.sys[system_of(q.sys_slot, q.n_sys, active ? (uint32_t)t : 0u)];
-    p.R_n = S.R_n; p.R_o = S.R_o; p.R_b = S.R_b;
-    p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
-    p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
-    p.seed_lo = S.seed_lo; p.seed_hi = S.seed_hi;
-    p.gf = S.g;
-    id0 = S.slot;
-  }
   bool outside = false;
   const int cx = cell_coord(rx, p.gf, outside), cy = cell_coord(ry, p.gf, outside),
             cz = cell_coord(rz, p.gf, outside);
@@ -802,7 +770,6 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
                     GZ = axis_gaps<R>(rz - (float)(cz - p.gf.half) * p.gf.csl, p.gf.csl);
 
   // rows sub, sub + 8, ...: trimmed slot ranges and the LBS sites of LAD beads
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   uint32_t lo[RPL], len[RPL], tot = 0;
 #pragma unroll
   for (int m = 0; m < RPL; ++m) {
@@ -830,7 +797,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
   if (listed) {
 #pragma unroll
     for (int m = 0; m < RPL; ++m)
-      for (uint32_t k = 0; k < len[m]; ++k) s_list[grp][off++] = lo[m] + k;
+      for (uint32_t k = 0; k < len[m]; ++k) list[off++] = lo[m] + k;
   }
   // the list is read by the lanes of the same wavefront only: complete the
   // LDS writes and order them before the reads, without a block barrier
@@ -838,18 +805,14 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-  if (EPI && active && sub == 0) {
-    bonded_force(p, q.pos, i, w, rx, ry, rz, fx, fy, fz);
-    wall_force<MODE>(p, rx, ry, rz, w, fx, fy, fz);
-  }
   if (listed) {
     uint32_t n = sub;
     for (; n + C_LANES < total; n += 2 * C_LANES) {
-      const float4 a = q.fspos[s_list[grp][n]], b = q.fspos[s_list[grp][n + C_LANES]];
+      const float4 a = q.fspos[list[n]], b = q.fspos[list[n + C_LANES]];
       pair_full<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
       pair_full<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
     }
-    if (n < total) pair_full<MODE>(p, rx, ry, rz, w, q.fspos[s_list[grp][n]], fx, fy, fz);
+    if (n < total) pair_full<MODE>(p, rx, ry, rz, w, q.fspos[list[n]], fx, fy, fz);
   } else {
     // crowded neighbourhood: each lane scans its own rows
 #pragma unroll
@@ -862,11 +825,105 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
     fy += __shfl_xor(fy, o);
     fz += __shfl_xor(fz, o);
   }
-  if (!active || sub != 0) return;
-  if constexpr (!EPI) {
-    q.fpair[i] = make_float4(fx, fy, fz, 0.0f);
+  // every lane's list reads are done before the group's next list is written
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ROUNDS = 1: each wavefront takes eight beads (eight lanes each); lane 0 of
+// each bead adds bonds and wall and runs the stage (het beads first within a
+// workgroup when p.group is set).
+// ROUNDS > 1: each wavefront runs the pair phase for ROUNDS x 8 beads, eight
+// at a time; lane 8 g + r keeps the total of bead g of round r, so the
+// per-bead work after the pair sum (bonds, bending, wall, noise, update,
+// binning) runs on ROUNDS x 8 lanes at once instead of 8.
+// EPI = false (ROUNDS = 1 only): the kernel stops after the pair sum and
+// writes it to q.fpair; k_integrate (one lane per bead) adds bonds and wall
+// and runs the stage.
+template <int MODE, int STAGE, bool BATCH, int R, bool EPI = true, int ROUNDS = 1>
+__global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
+  static_assert(ROUNDS == 1 || EPI, "the split integration pass takes one round");
+  static_assert(ROUNDS >= 1 && ROUNDS <= C_LANES, "a group keeps at most one bead per lane");
+  __shared__ uint32_t s_list[C_BEADS_PER_BLOCK][C_LIST];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const int sub = lane % C_LANES, gw = lane / C_LANES, grp = wave * (64 / C_LANES) + gw;
+  const int wbase = (xcd_block() * (BEAD_BLOCK / 64) + wave) * ((64 / C_LANES) * ROUNDS);
+  if constexpr (ROUNDS == 1) {
+    int t = wbase + gw;
+    if (p.group) {
+      // A wavefront runs as long as its longest list. Het beads (cutoff 2.5
+      // sigma) have about four times the candidates of the others, so the
+      // workgroup's slots are reordered: het slots first, then the rest, each
+      // in cell order. Position grp takes the k-th set bit of the het mask.
+      __shared__ uint32_t s_mask[BEAD_BLOCK / 64];
+      const int t0 = wbase - wave * (64 / C_LANES);
+      const bool het = t < p.N && (wbits(q.fspos[t]) & W_HET) != 0u;
+      const uint64_t b = __ballot(het && sub == 0);
+      uint32_t m8 = 0;
+#pragma unroll
+      for (int k = 0; k < 64 / C_LANES; ++k) m8 |= (uint32_t)((b >> (k * C_LANES)) & 1u) << k;
+      if (lane == 0) s_mask[wave] = m8;
+      __syncthreads();
+      uint32_t m = 0;
+#pragma unroll
+      for (int k = 0; k < BEAD_BLOCK / 64; ++k) m |= s_mask[k] << (k * (64 / C_LANES));
+      const int nh = __popc(m);
+      uint32_t mm = grp < nh ? m : ~m;
+      int k = grp < nh ? grp : grp - nh, pos = 0;
+#pragma unroll
+      for (int sh = C_BEADS_PER_BLOCK / 2; sh > 0; sh >>= 1) {
+        const int c = __popc(mm & ((1u << sh) - 1u));
+        if (k >= c) {
+          k -= c;
+          mm >>= sh;
+          pos += sh;
+        }
+      }
+      t = t0 + pos;
+    }
+    const bool active = t < p.N;
+    const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t w = wbits(me);
+    const uint32_t i = w & ID_MASK;
+    if constexpr (BATCH) load_system(p, q, active ? (uint32_t)t : 0u);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (EPI && active && sub == 0) {
+      bonded_force(p, q.pos, i, w, me.x, me.y, me.z, fx, fy, fz);
+      wall_force<MODE>(p, me.x, me.y, me.z, w, fx, fy, fz);
+    }
+    bal_pair_sum<MODE, BATCH, R>(p, q, s_list[grp], active, sub, lane, me, fx, fy, fz);
+    if (!active || sub != 0) return;
+    if constexpr (!EPI) {
+      q.fpair[i] = make_float4(fx, fy, fz, 0.0f);
+    } else {
+      stage_epilogue<STAGE>(p, q, i, w, me, fx, fy, fz);
+    }
   } else {
-    stage_epilogue<STAGE>(p, q, i, id0, w, me, fx, fy, fz);
+    float kx = 0.0f, ky = 0.0f, kz = 0.0f;
+    for (int rd = 0; rd < ROUNDS; ++rd) {
+      const int t = wbase + rd * (64 / C_LANES) + gw;
+      const bool active = t < p.N;
+      const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if constexpr (BATCH) load_system(p, q, active ? (uint32_t)t : 0u);
+      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      bal_pair_sum<MODE, BATCH, R>(p, q, s_list[grp], active, sub, lane, me, fx, fy, fz);
+      if (sub == rd) {
+        kx = fx;
+        ky = fy;
+        kz = fz;
+      }
+    }
+    if (sub >= ROUNDS) return;
+    const int t = wbase + sub * (64 / C_LANES) + gw;
+    if (t >= p.N) return;
+    const float4 me = q.fspos[t];
+    const uint32_t w = wbits(me);
+    const uint32_t i = w & ID_MASK;
+    if constexpr (BATCH) load_system(p, q, (uint32_t)t);
+    bonded_force(p, q.pos, i, w, me.x, me.y, me.z, kx, ky, kz);
+    wall_force<MODE>(p, me.x, me.y, me.z, w, kx, ky, kz);
+    stage_epilogue<STAGE>(p, q, i, w, me, kx, ky, kz);
   }
 }
 
@@ -879,21 +936,12 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_integrate(KParams p, KPtrs q) {
   const uint32_t i = (uint32_t)t;
   const float4 me = q.pos[i];
   const uint32_t w = wbits(me);
-  uint32_t id0 = 0;
-  if constexpr (BATCH) {
-    const SysParams &S = q.sys[system_of(q.sys_slot, q.n_sys, i)];
-    p.R_n = S.R_n; p.R_o = S.R_o; p.R_b = S.R_b;
-    p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
-    p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
-    p.seed_lo = S.seed_lo; p.seed_hi = S.seed_hi;
-    p.gf = S.g;
-    id0 = S.slot;
-  }
+  if constexpr (BATCH) load_system(p, q, i);
   const float4 fp = q.fpair[i];
   float fx = fp.x, fy = fp.y, fz = fp.z;
   bonded_force(p, q.pos, i, w, me.x, me.y, me.z, fx, fy, fz);
   wall_force<MODE>(p, me.x, me.y, me.z, w, fx, fy, fz);
-  stage_epilogue<STAGE>(p, q, i, id0, w, me, fx, fy, fz);
+  stage_epilogue<STAGE>(p, q, i, w, me, fx, fy, fz);
 }
 
 // Binning of an index-ordered array (initial builds, LBS grid, export grid).
@@ -927,7 +975,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_bin_batch(const SysParams *sys, 
 constexpr int ORDER_LDS = 4096;
 
 __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, uint32_t *flags,
-                                                      uint64_t *step_inc) {
+                                                      uint64_t *step_inc, NoiseJob nj) {
   __shared__ uint32_t red[SCAN_BLOCK / 64], wsum[SCAN_BLOCK / 64], msum[SCAN_BLOCK / 64];
   __shared__ uint32_t s_id[ORDER_LDS];
   __shared__ uint8_t s_cell[ORDER_LDS];
@@ -1007,6 +1055,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
       if (J.soi) J.soi[start + rank] = e;
     }
   }
+  // the next step's normals: one lane per bead at full width, where the
+  // force kernels would draw them on one lane of eight
+  if (nj.z)
+    for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < nj.n; g += gridDim.x * SCAN_BLOCK) fill_noise(nj, g);
+}
+
+__global__ __launch_bounds__(BEAD_BLOCK) void k_noise_fill(NoiseJob nj) {
+  const uint32_t g = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (g < nj.n) fill_noise(nj, g);
 }
 
 // Contact counts on the fine grid: pairs i < j with d < rc within
@@ -1233,10 +1290,18 @@ void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, c
                                                             bucket, flags);
 }
 
-void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc) {
+void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc,
+                  const NoiseJob *noise) {
   const OrderJob jb = b ? *b : a;
   const unsigned blocks = (unsigned)(a.g.nblk + (b ? b->g.nblk : 0));
-  k_order<<<blocks, SCAN_BLOCK, 0, s>>>(a, jb, flags, step_inc);
+  NoiseJob nj{};
+  if (noise) nj = *noise;
+  k_order<<<blocks, SCAN_BLOCK, 0, s>>>(a, jb, flags, step_inc, nj);
+}
+
+void launch_noise_fill(hipStream_t s, const NoiseJob &nj) {
+  if (nj.n == 0) return;
+  k_noise_fill<<<blocks_for((int)nj.n, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(nj);
 }
 
 template <int MODE, bool SPLIT, bool BATCH, int R, int LANES>
@@ -1257,6 +1322,27 @@ static void force_balanced(hipStream_t s, int stage, const KParams &p, const KPt
     if (stage == 0) k_integrate<MODE, 0, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
     else if (stage == 1) k_integrate<MODE, 1, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
     else k_integrate<MODE, 2, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
+    return;
+  }
+  if (p.rounds == 4) {
+    const unsigned g4 = blocks_for(p.N, C_BEADS_PER_BLOCK * 4);
+    if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 4><<<g4, BEAD_BLOCK, 0, s>>>(p, q);
+    else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 4><<<g4, BEAD_BLOCK, 0, s>>>(p, q);
+    else k_force_bal<MODE, 2, BATCH, R, true, 4><<<g4, BEAD_BLOCK, 0, s>>>(p, q);
+    return;
+  }
+  if (p.rounds == 2) {
+    const unsigned g2 = blocks_for(p.N, C_BEADS_PER_BLOCK * 2);
+    if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 2><<<g2, BEAD_BLOCK, 0, s>>>(p, q);
+    else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 2><<<g2, BEAD_BLOCK, 0, s>>>(p, q);
+    else k_force_bal<MODE, 2, BATCH, R, true, 2><<<g2, BEAD_BLOCK, 0, s>>>(p, q);
+    return;
+  }
+  if (p.rounds == 8) {
+    const unsigned g8 = blocks_for(p.N, C_BEADS_PER_BLOCK * 8);
+    if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 8><<<g8, BEAD_BLOCK, 0, s>>>(p, q);
+    else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 8><<<g8, BEAD_BLOCK, 0, s>>>(p, q);
+    else k_force_bal<MODE, 2, BATCH, R, true, 8><<<g8, BEAD_BLOCK, 0, s>>>(p, q);
     return;
   }
   if (stage == 0) k_force_bal<MODE, 0, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);

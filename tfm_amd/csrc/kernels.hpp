@@ -70,6 +70,7 @@ struct KParams {
   int reach;   // neighbour cells scanned on each side: 1 (27 cells) or 2 (125 cells of half the side)
   int lanes;   // lanes per bead of the force kernel: 2 reach + 1 (one per z-plane), 1, or 8 (balanced list)
   int group;   // 8 lanes: within a workgroup, beads with the long (het) cutoff take the first wavefronts
+  int rounds;  // 8 lanes: beads per lane group of a wavefront (1, 2, 4 or 8), eight per round of the pair phase
   // nucleus geometry (cngeom, chrdat.cuh:48-69)
   float R_n, R_o, R_b, noc, boc, nod, bod;
   // bonds and bending
@@ -110,6 +111,7 @@ struct KPtrs {
   uint32_t *flags;         // FLAG_COUNT error counters
   const uint64_t *step;    // global step counter (noise counter)
   float4 *fpair;           // pair forces by bead index (balanced kernel with a separate integration pass)
+  const float4 *noise;     // unit normal triples of this step by bead index (first Heun stage)
   // batches only (sys == nullptr for a single system)
   const SysParams *sys;    // n_sys systems
   const uint32_t *sys_slot; // n_sys + 1 first slots
@@ -129,10 +131,26 @@ struct OrderJob {
   uint32_t *cstart;
 };
 
+// Unit normal triples of one step for every bead, by bead index: Philox
+// counter {local bead, step, attempt} under the bead's system key. The order
+// pass after the first Heun stage fills them for the next step (ahead = 1).
+struct NoiseJob {
+  float4 *z;               // out: (z0, z1, z2, attempts) per bead
+  const uint64_t *step;    // device step counter
+  uint32_t ahead;          // noise of step *step + ahead
+  uint32_t n;              // beads
+  uint32_t seed_lo, seed_hi;
+  const SysParams *sys;    // batch: per-system keys and first beads (nullptr: one system)
+  const uint32_t *sys_slot;
+  int n_sys;
+};
+
 // Launchers (kernels.hip). All are asynchronous on `s`.
 void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, const float4 *pos, bool het_only,
                 uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags);
-void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc);
+void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc,
+                  const NoiseJob *noise = nullptr);
+void launch_noise_fill(hipStream_t s, const NoiseJob &nj);
 void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q);
 // Contacts of the beads in slots [slot0, slot0 + p.N) (bead indices from
 // id0) on p.gf, binned by (index - id0) / contact_bin_size.

@@ -112,6 +112,7 @@ struct ccp_sim {
   uint32_t lanes = 3; // lanes per bead of the force kernel
   bool integrate_pass = false; // 8 lanes: bonds, wall and stage update in a one-lane-per-bead pass
   bool group_types = true;     // 8 lanes: het beads first within each workgroup
+  uint32_t rounds = 1;         // 8 lanes: pair-phase rounds per wavefront before the per-bead work
   float fcap = 0.0f;
   KParams kp{};
 
@@ -123,11 +124,14 @@ struct ccp_sim {
   std::vector<uint8_t> hfl;
   bool lgrid_valid = false, dstep_dirty = true;
   uint32_t n_samples = 0;
+  // step whose normals znoise holds (the first Heun stage reads them; the
+  // order pass after it writes the next step's)
+  uint64_t noise_step = ~0ull;
 
   // device
   int dev = 0;
   hipStream_t stream = nullptr;
-  DevBuf<float4> r, er, pstar, lr, lspos, aux, fpair;
+  DevBuf<float4> r, er, pstar, lr, lspos, aux, fpair, znoise;
   DevBuf<uint32_t> lstart, flags, cmap;
   DevBuf<uint64_t> dstep;
   DevBuf<unsigned long long> dcount;
@@ -181,7 +185,7 @@ struct ccp_sim {
 
   ~ccp_sim() {
     drop_graphs();
-    DevBuf<float4> *f4[] = {&r, &er, &pstar, &lr, &lspos, &aux, &fpair};
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &lr, &lspos, &aux, &fpair, &znoise};
     for (auto *b : f4) b->free_();
     DevBuf<uint32_t> *u4[] = {&lstart, &flags, &cmap};
     for (auto *b : u4) b->free_();
@@ -236,7 +240,18 @@ struct ccp_sim {
   void ensure_grids();
   void invalidate() { gf.valid = gh.valid = gx.valid = false; }
   void sync_step();
-  void launch_stage(int mode, int stage);
+  NoiseJob noise_job(uint32_t ahead) const {
+    return NoiseJob{znoise.p, dstep.p, ahead, N, kp.seed_lo, kp.seed_hi, nullptr, nullptr, 0};
+  }
+  // before the first stage of a run: normals of the current step
+  void ensure_noise() {
+    sync_step();
+    if (noise_step == step) return;
+    launch_noise_fill(stream, noise_job(0));
+    noise_step = step;
+  }
+  // `mid` (optional) is recorded between the force and the order launches
+  void launch_stage(int mode, int stage, hipEvent_t mid = nullptr);
   void launch_step(int mode) {
     launch_stage(mode, 1);
     launch_stage(mode, 2);
