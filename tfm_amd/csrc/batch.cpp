@@ -75,7 +75,7 @@ struct ccp_batch {
   // device
   int dev = 0;
   hipStream_t stream = nullptr;
-  DevBuf<float4> r, er, pstar, fpair, znoise;
+  DevBuf<float4> r, er, pstar, fpair, znoise, fbond;
   DevBuf<SysParams> dsys;
   DevBuf<uint32_t> dslot, flags, cmaps;
   DevBuf<uint64_t> dstep;
@@ -93,7 +93,7 @@ struct ccp_batch {
   }
   ~ccp_batch() {
     drop_graphs();
-    DevBuf<float4> *f4[] = {&r, &er, &pstar, &fpair, &znoise};
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &fpair, &znoise, &fbond};
     for (auto *b : f4) b->free_();
     DevBuf<uint32_t> *u4[] = {&dslot, &flags, &cmaps};
     for (auto *b : u4) b->free_();
@@ -182,10 +182,17 @@ struct ccp_batch {
   NoiseJob noise_job(uint32_t ahead) const {
     return NoiseJob{znoise.p, dstep.p, ahead, N, 0u, 0u, dsys.p, dslot.p, (int)n_sys};
   }
-  void ensure_noise() {
+  bool bond_pass() const { return kp.lanes == 8 && !integrate_pass; }
+  BondJob bond_job(const float4 *pos) const {
+    return BondJob{fbond.p, pos, N, MODE_LJ, dsys.p, dslot.p, (int)n_sys};
+  }
+  // before the first stage of a run: the current step's normals (unless held
+  // already) and the bonded terms of the current positions
+  void prime_terms() {
     sync_step();
-    if (noise_step == step) return;
-    launch_noise_fill(stream, noise_job(0));
+    const NoiseJob nj = noise_job(0);
+    const BondJob bj = bond_job(r.p);
+    launch_bead_terms(stream, noise_step != step ? &nj : nullptr, kp, bond_pass() ? &bj : nullptr);
     noise_step = step;
   }
   void launch_stage(int stage) {
@@ -202,14 +209,19 @@ struct ccp_batch {
     q.step = dstep.p;
     q.fpair = fpair.p;
     q.noise = znoise.p;
+    q.fbond = bond_pass() ? fbond.p : nullptr;
     q.sys = dsys.p;
     q.sys_slot = dslot.p;
     q.n_sys = (int)n_sys;
     launch_force(stream, MODE_LJ, stage, kp, q);
     OrderJob j{grid.g, grid.count.p, grid.bs(grid.clean), grid.bs(grid.clean ^ 1), grid.bucket.p, q.out0,
                grid.spos.p, nullptr, grid.cstart.p};
+    // with the order pass: the next step's normals after the first stage,
+    // the bonded terms of q.out0 after both
     const NoiseJob nj = noise_job(1);
-    launch_order(stream, j, nullptr, flags.p, stage == 2 ? dstep.p : nullptr, stage == 1 ? &nj : nullptr);
+    const BondJob bj = bond_job(q.out0);
+    launch_order(stream, j, nullptr, flags.p, stage == 2 ? dstep.p : nullptr, stage == 1 ? &nj : nullptr, &kp,
+                 bond_pass() ? &bj : nullptr);
     grid.clean ^= 1;
   }
   void run_chunk(uint64_t n) {
@@ -267,7 +279,7 @@ struct ccp_batch {
   }
   void run_steps(uint64_t n) {
     ensure_grid();
-    ensure_noise();
+    prime_terms();
     while (n > 0) {
       uint64_t chunk = n;
       if (cinterval) chunk = std::min<uint64_t>(n, cinterval - step % cinterval);
@@ -398,6 +410,7 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
     b->er.alloc(b->N);
     b->pstar.alloc(b->N);
     b->znoise.alloc(b->N);
+    if (b->bond_pass()) b->fbond.alloc(b->N);
     if (b->kp.lanes == 8 && b->integrate_pass) b->fpair.alloc(b->N);
     b->dsys.alloc(n_sys);
     b->dslot.alloc(n_sys + 1);
@@ -534,7 +547,7 @@ int ccp_batch_time_steps(ccp_batch *b, uint32_t n, float *ms_per_step) {
   return bguarded(b, [&] {
     bneed(ms_per_step && n > 0, "bad arguments");
     b->ensure_grid();
-    b->ensure_noise();
+    b->prime_terms();
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0));
     hip_check(hipEventCreate(&e1));

@@ -261,6 +261,7 @@ void ccp_sim::allocate() {
   hip_check(hipGetDevice(&dev));
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   r.alloc(N); er.alloc(N); pstar.alloc(N); aux.alloc(N); znoise.alloc(N);
+  if (bond_pass()) fbond.alloc(N);
   if (lanes == 8 && integrate_pass) fpair.alloc(N);
   lr.alloc(n_l); lspos.alloc(n_l); lstart.alloc(gf.g.n_c + 1);
   flags.alloc(FLAG_COUNT);
@@ -426,15 +427,19 @@ void ccp_sim::launch_stage(int mode, int stage, hipEvent_t mid) {
   q.step = dstep.p;
   q.fpair = fpair.p;
   q.noise = znoise.p;
+  q.fbond = bond_pass() ? fbond.p : nullptr;
   launch_force(stream, mode, stage, k, q);
   if (mid) hip_check(hipEventRecord(mid, stream));
   OrderJob jf{gf.g, gf.count.p, gf.bs(gf.clean), gf.bs(gf.clean ^ 1), gf.bucket.p, q.out0, gf.spos.p, nullptr,
               gf.cstart.p};
   OrderJob jh{gh.g, gh.count.p, gh.bs(gh.clean), gh.bs(gh.clean ^ 1), gh.bucket.p, q.out0, gh.spos.p, nullptr,
               gh.cstart.p};
-  // after the first stage the counter still holds this step: draw the next one's
+  // With the order pass: after the first stage the counter still holds this
+  // step, so draw the next one's normals; the bonded terms of q.out0.
   const NoiseJob nj = noise_job(1);
-  launch_order(stream, jf, het ? &jh : nullptr, flags.p, stage == 2 ? dstep.p : nullptr, stage == 1 ? &nj : nullptr);
+  const BondJob bj = bond_job(q.out0, mode);
+  launch_order(stream, jf, het ? &jh : nullptr, flags.p, stage == 2 ? dstep.p : nullptr, stage == 1 ? &nj : nullptr,
+               &k, bond_pass() ? &bj : nullptr);
   gf.clean ^= 1;
   if (het) gh.clean ^= 1;
   else gh.valid = false;
@@ -472,7 +477,7 @@ void ccp_sim::run_chunk(uint64_t n, int mode) {
 
 void ccp_sim::run_steps(uint64_t n, int mode) {
   ensure_grids();
-  ensure_noise();
+  prime_terms(mode);
   while (n > 0) {
     uint64_t chunk = n;
     if (cinterval && mode != MODE_ICG) chunk = std::min<uint64_t>(n, cinterval - step % cinterval);
@@ -1288,7 +1293,7 @@ int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]) {
     need(out && n_steps > 0, "bad timing request");
     const int mode = h->ff == CCP_FF_LJ ? MODE_LJ : MODE_CCP;
     h->ensure_grids();
-    h->ensure_noise();
+    h->prime_terms(mode);
     hipEvent_t ev[5];
     for (auto &e : ev) hip_check(hipEventCreate(&e));
     double acc[4] = {0, 0, 0, 0};

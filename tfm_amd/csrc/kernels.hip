@@ -542,6 +542,24 @@ __device__ __forceinline__ void fill_noise(const NoiseJob &nj, uint32_t g) {
   nj.z[g] = make_float4(z0, z1, z2, __uint_as_float(a));
 }
 
+// Bonded + wall force of bead g for the next force pass (one lane per bead).
+__device__ __forceinline__ void fill_bond(KParams p, const BondJob &bj, uint32_t g) {
+  if (bj.sys) {
+    const SysParams &S = bj.sys[system_of(bj.sys_slot, bj.n_sys, g)];
+    p.R_n = S.R_n; p.R_o = S.R_o; p.R_b = S.R_b;
+    p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
+    p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
+  }
+  const float4 me = bj.pos[g];
+  const uint32_t w = wbits(me);
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  bonded_force(p, bj.pos, g, w, me.x, me.y, me.z, fx, fy, fz);
+  if (bj.mode == MODE_LJ) wall_force<MODE_LJ>(p, me.x, me.y, me.z, w, fx, fy, fz);
+  else if (bj.mode == MODE_ICG) wall_force<MODE_ICG>(p, me.x, me.y, me.z, w, fx, fy, fz);
+  else wall_force<MODE_CCP>(p, me.x, me.y, me.z, w, fx, fy, fz);
+  bj.out[g] = make_float4(fx, fy, fz, 0.0f);
+}
+
 // Force evaluation of one stage, beads in grid order.
 // STAGE 0: write F(pos) by bead index. STAGE 1: Heun predictor
 // (exec_RK_1, chrsim.cu:364-388) er = r + f dt + rn, keeps
@@ -889,8 +907,13 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
     if constexpr (BATCH) load_system(p, q, active ? (uint32_t)t : 0u);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (EPI && active && sub == 0) {
-      bonded_force(p, q.pos, i, w, me.x, me.y, me.z, fx, fy, fz);
-      wall_force<MODE>(p, me.x, me.y, me.z, w, fx, fy, fz);
+      if (q.fbond) {
+        const float4 fb = q.fbond[i];
+        fx = fb.x; fy = fb.y; fz = fb.z;
+      } else {
+        bonded_force(p, q.pos, i, w, me.x, me.y, me.z, fx, fy, fz);
+        wall_force<MODE>(p, me.x, me.y, me.z, w, fx, fy, fz);
+      }
     }
     bal_pair_sum<MODE, BATCH, R>(p, q, s_list[grp], active, sub, lane, me, fx, fy, fz);
     if (!active || sub != 0) return;
@@ -921,8 +944,13 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
     const uint32_t w = wbits(me);
     const uint32_t i = w & ID_MASK;
     if constexpr (BATCH) load_system(p, q, (uint32_t)t);
-    bonded_force(p, q.pos, i, w, me.x, me.y, me.z, kx, ky, kz);
-    wall_force<MODE>(p, me.x, me.y, me.z, w, kx, ky, kz);
+    if (q.fbond) {
+      const float4 fb = q.fbond[i];
+      kx += fb.x; ky += fb.y; kz += fb.z;
+    } else {
+      bonded_force(p, q.pos, i, w, me.x, me.y, me.z, kx, ky, kz);
+      wall_force<MODE>(p, me.x, me.y, me.z, w, kx, ky, kz);
+    }
     stage_epilogue<STAGE>(p, q, i, w, me, kx, ky, kz);
   }
 }
@@ -975,7 +1003,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_bin_batch(const SysParams *sys, 
 constexpr int ORDER_LDS = 4096;
 
 __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, uint32_t *flags,
-                                                      uint64_t *step_inc, NoiseJob nj) {
+                                                      uint64_t *step_inc, NoiseJob nj, KParams bp, BondJob bj) {
   __shared__ uint32_t red[SCAN_BLOCK / 64], wsum[SCAN_BLOCK / 64], msum[SCAN_BLOCK / 64];
   __shared__ uint32_t s_id[ORDER_LDS];
   __shared__ uint8_t s_cell[ORDER_LDS];
@@ -1055,15 +1083,21 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
       if (J.soi) J.soi[start + rank] = e;
     }
   }
-  // the next step's normals: one lane per bead at full width, where the
-  // force kernels would draw them on one lane of eight
+  // Per-bead terms of the next force pass at one lane per bead, where the
+  // force kernels would run them on one lane of eight: the next step's
+  // normals (nj.z) and the bonded + wall forces of the positions just
+  // ordered (bj.out). The first N / 256 workgroups take them, in bead order.
   if (nj.z)
     for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < nj.n; g += gridDim.x * SCAN_BLOCK) fill_noise(nj, g);
+  if (bj.out)
+    for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < bj.n; g += gridDim.x * SCAN_BLOCK) fill_bond(bp, bj, g);
 }
 
-__global__ __launch_bounds__(BEAD_BLOCK) void k_noise_fill(NoiseJob nj) {
+// The same terms on their own, before the first stage of a run.
+__global__ __launch_bounds__(BEAD_BLOCK) void k_bead_terms(NoiseJob nj, KParams p, BondJob bj) {
   const uint32_t g = blockIdx.x * BEAD_BLOCK + threadIdx.x;
-  if (g < nj.n) fill_noise(nj, g);
+  if (nj.z && g < nj.n) fill_noise(nj, g);
+  if (bj.out && g < bj.n) fill_bond(p, bj, g);
 }
 
 // Contact counts on the fine grid: pairs i < j with d < rc within
@@ -1291,17 +1325,28 @@ void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, c
 }
 
 void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc,
-                  const NoiseJob *noise) {
+                  const NoiseJob *noise, const KParams *bp, const BondJob *bond) {
   const OrderJob jb = b ? *b : a;
   const unsigned blocks = (unsigned)(a.g.nblk + (b ? b->g.nblk : 0));
   NoiseJob nj{};
+  KParams p{};
+  BondJob bj{};
   if (noise) nj = *noise;
-  k_order<<<blocks, SCAN_BLOCK, 0, s>>>(a, jb, flags, step_inc, nj);
+  if (bond && bp) {
+    p = *bp;
+    bj = *bond;
+  }
+  k_order<<<blocks, SCAN_BLOCK, 0, s>>>(a, jb, flags, step_inc, nj, p, bj);
 }
 
-void launch_noise_fill(hipStream_t s, const NoiseJob &nj) {
-  if (nj.n == 0) return;
-  k_noise_fill<<<blocks_for((int)nj.n, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(nj);
+void launch_bead_terms(hipStream_t s, const NoiseJob *noise, const KParams &p, const BondJob *bond) {
+  NoiseJob nj{};
+  BondJob bj{};
+  if (noise) nj = *noise;
+  if (bond) bj = *bond;
+  const uint32_t n = nj.z ? nj.n : bj.out ? bj.n : 0u; // both cover every bead
+  if (n == 0) return;
+  k_bead_terms<<<blocks_for((int)n, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(nj, p, bj);
 }
 
 template <int MODE, bool SPLIT, bool BATCH, int R, int LANES>

@@ -112,6 +112,7 @@ struct KPtrs {
   const uint64_t *step;    // global step counter (noise counter)
   float4 *fpair;           // pair forces by bead index (balanced kernel with a separate integration pass)
   const float4 *noise;     // unit normal triples of this step by bead index (first Heun stage)
+  const float4 *fbond;     // bonds + bending + wall per bead by bead index (balanced kernel; nullptr: inline)
   // batches only (sys == nullptr for a single system)
   const SysParams *sys;    // n_sys systems
   const uint32_t *sys_slot; // n_sys + 1 first slots
@@ -145,12 +146,28 @@ struct NoiseJob {
   int n_sys;
 };
 
+// Bonded (bonds, bending) and wall forces of the positions the next force
+// pass evaluates, one lane per bead, written by bead index. The order pass
+// after each stage fills them for the next stage.
+struct BondJob {
+  float4 *out;              // out: force per bead (nullptr: none)
+  const float4 *pos;        // positions by bead index
+  uint32_t n;               // beads
+  int mode;                 // pair set of the wall (MODE_*)
+  const SysParams *sys;     // batch: per-system geometry (nullptr: one system)
+  const uint32_t *sys_slot;
+  int n_sys;
+};
+
 // Launchers (kernels.hip). All are asynchronous on `s`.
 void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, const float4 *pos, bool het_only,
                 uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags);
+// The order pass also fills the next force pass's per-bead terms when given:
+// normals (noise) and bonded + wall forces (bp, bond).
 void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc,
-                  const NoiseJob *noise = nullptr);
-void launch_noise_fill(hipStream_t s, const NoiseJob &nj);
+                  const NoiseJob *noise = nullptr, const KParams *bp = nullptr, const BondJob *bond = nullptr);
+// the same per-bead terms on their own (before the first stage of a run)
+void launch_bead_terms(hipStream_t s, const NoiseJob *noise, const KParams &p, const BondJob *bond);
 void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q);
 // Contacts of the beads in slots [slot0, slot0 + p.N) (bead indices from
 // id0) on p.gf, binned by (index - id0) / contact_bin_size.

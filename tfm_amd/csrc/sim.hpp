@@ -131,7 +131,7 @@ struct ccp_sim {
   // device
   int dev = 0;
   hipStream_t stream = nullptr;
-  DevBuf<float4> r, er, pstar, lr, lspos, aux, fpair, znoise;
+  DevBuf<float4> r, er, pstar, lr, lspos, aux, fpair, znoise, fbond;
   DevBuf<uint32_t> lstart, flags, cmap;
   DevBuf<uint64_t> dstep;
   DevBuf<unsigned long long> dcount;
@@ -185,7 +185,7 @@ struct ccp_sim {
 
   ~ccp_sim() {
     drop_graphs();
-    DevBuf<float4> *f4[] = {&r, &er, &pstar, &lr, &lspos, &aux, &fpair, &znoise};
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &lr, &lspos, &aux, &fpair, &znoise, &fbond};
     for (auto *b : f4) b->free_();
     DevBuf<uint32_t> *u4[] = {&lstart, &flags, &cmap};
     for (auto *b : u4) b->free_();
@@ -243,11 +243,20 @@ struct ccp_sim {
   NoiseJob noise_job(uint32_t ahead) const {
     return NoiseJob{znoise.p, dstep.p, ahead, N, kp.seed_lo, kp.seed_hi, nullptr, nullptr, 0};
   }
-  // before the first stage of a run: normals of the current step
-  void ensure_noise() {
+  // balanced kernel: bonds, bending and wall come from fbond, which the order
+  // pass after each stage fills for the next one
+  bool bond_pass() const { return lanes == 8 && !integrate_pass; }
+  BondJob bond_job(const float4 *pos, int mode) const {
+    return BondJob{fbond.p, pos, N, mode, nullptr, nullptr, 0};
+  }
+  // before the first stage of a run: the normals of the current step (unless
+  // held already) and the bonded terms of the current positions
+  void prime_terms(int mode) {
     sync_step();
-    if (noise_step == step) return;
-    launch_noise_fill(stream, noise_job(0));
+    const NoiseJob nj = noise_job(0);
+    const BondJob bj = bond_job(r.p, mode);
+    const KParams k = mode_params(mode);
+    launch_bead_terms(stream, noise_step != step ? &nj : nullptr, k, bond_pass() ? &bj : nullptr);
     noise_step = step;
   }
   // `mid` (optional) is recorded between the force and the order launches
