@@ -769,6 +769,25 @@ __device__ __forceinline__ void trimmed_row(const GridGeom &g, int cx, int cy, i
 }
 
 
+// Reach 2: the 25 neighbour rows (dy, dz) in ring order, the 3 x 3 inner
+// rows first. A cutoff shorter than the cell side never reaches the outer
+// ring, and a wavefront whose beads all have such cutoffs skips it.
+constexpr int RING2[25][2] = {{0, 0},   {-1, 0}, {1, 0},  {0, -1}, {0, 1},  {-1, -1}, {1, -1},
+                              {-1, 1},  {1, 1},  {-2, -2}, {-1, -2}, {0, -2}, {1, -2},  {2, -2},
+                              {-2, -1}, {2, -1}, {-2, 0}, {2, 0},  {-2, 1}, {2, 1},   {-2, 2},
+                              {-1, 2},  {0, 2},  {1, 2},  {2, 2}};
+// rows 8 m .. 8 m + 7, one per lane of a bead's group: 6 bits each,
+// (dy + 2) | (dz + 2) << 3
+constexpr uint64_t ring2_word(int m) {
+  uint64_t w = 0;
+  for (int k = 0; k < C_LANES; ++k) {
+    const int r = m * C_LANES + k;
+    if (r < 25) w |= (uint64_t)((RING2[r][0] + 2) | ((RING2[r][1] + 2) << 3)) << (6 * k);
+  }
+  return w;
+}
+constexpr uint64_t RING2_WORDS[4] = {ring2_word(0), ring2_word(1), ring2_word(2), ring2_word(3)};
+
 // Pair (and LBS) forces of the bead `me` over its trimmed candidate list,
 // spread over the bead's eight lanes (sub = 0..7) and added to f; on return
 // every lane of the group holds the bead's total. `list` is the group's LDS
@@ -787,13 +806,30 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
                     GY = axis_gaps<R>(ry - (float)(cy - p.gf.half) * p.gf.csl, p.gf.csl),
                     GZ = axis_gaps<R>(rz - (float)(cz - p.gf.half) * p.gf.csl, p.gf.csl);
 
-  // rows sub, sub + 8, ...: trimmed slot ranges and the LBS sites of LAD beads
+  // Reach 2: rows 16.. (the outer ring only) are skipped by a wavefront in
+  // which no bead's cutoff reaches a cell two steps away in y or z.
+  bool any_outer = true;
+  if constexpr (R == 2) {
+    const float near2 = fminf(fminf(GY.lo[1], GY.hi[1]), fminf(GZ.lo[1], GZ.hi[1]));
+    any_outer = __ballot(active && rc2 >= near2) != 0ull;
+  }
+  // rows sub, sub + 8, ... (reach 2: in ring order): trimmed slot ranges and
+  // the LBS sites of LAD beads
   uint32_t lo[RPL], len[RPL], tot = 0;
 #pragma unroll
   for (int m = 0; m < RPL; ++m) {
+    lo[m] = 0;
+    len[m] = 0;
+    if (R == 2 && m >= 2 && !any_outer) continue;
     const int r = sub + m * C_LANES;
+    int dy = r % W - R, dz = r / W - R;
+    if constexpr (R == 2) {
+      const uint32_t v = (uint32_t)(RING2_WORDS[m] >> (6 * sub)) & 63u;
+      dy = (int)(v & 7u) - 2;
+      dz = (int)(v >> 3) - 2;
+    }
     int c0, c1;
-    trimmed_row<R>(p.gf, cx, cy, cz, GX, GY, GZ, r % W - R, r / W - R, rc2, active && r < NROW, c0, c1);
+    trimmed_row<R>(p.gf, cx, cy, cz, GX, GY, GZ, dy, dz, rc2, active && r < NROW, c0, c1);
     lo[m] = q.fcstart[c0];
     len[m] = q.fcstart[c1] - lo[m];
     tot += len[m];
@@ -1014,6 +1050,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
   const OrderJob &J = second ? jb : ja;
   const int b = second ? bx - ja.g.nblk : bx;
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
+  // Per-bead terms of the next force pass at one lane per bead, where the
+  // force kernels would run them on one lane of eight: the next step's
+  // normals (nj.z) and the bonded + wall forces of the positions just
+  // ordered (bj.out). The first N / 256 workgroups take them, in bead order,
+  // ahead of the ordering so that their arithmetic covers its first loads.
+  if (nj.z)
+    for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < nj.n; g += gridDim.x * SCAN_BLOCK) fill_noise(nj, g);
+  if (bj.out)
+    for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < bj.n; g += gridDim.x * SCAN_BLOCK) fill_bond(bp, bj, g);
   const int c = b * SCAN_BLOCK + tid;
   const int cap = J.g.cap, n_c = J.g.n_c;
   uint32_t part = 0;
@@ -1083,14 +1128,6 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
       if (J.soi) J.soi[start + rank] = e;
     }
   }
-  // Per-bead terms of the next force pass at one lane per bead, where the
-  // force kernels would run them on one lane of eight: the next step's
-  // normals (nj.z) and the bonded + wall forces of the positions just
-  // ordered (bj.out). The first N / 256 workgroups take them, in bead order.
-  if (nj.z)
-    for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < nj.n; g += gridDim.x * SCAN_BLOCK) fill_noise(nj, g);
-  if (bj.out)
-    for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < bj.n; g += gridDim.x * SCAN_BLOCK) fill_bond(bp, bj, g);
 }
 
 // The same terms on their own, before the first stage of a run.
