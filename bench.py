@@ -57,16 +57,18 @@ PARAMS = {
 }
 
 # Algorithmic HBM bytes per bead for one launch of each step kernel
-# (16 B float4 positions, 4 B indices/counters; neighbour gathers counted
-# once because the sorted copy is read once per launch):
-#   force stage 1: R sorted pos 16 + R pos by index 16 (bonds) + W er 16
-#                  + W r+f dt/2+noise 16 + W bucket slot 4 + count atomic 4 = 72
-#   force stage 2: R sorted pos 16 + R er by index 16 + R r+f dt/2+noise 16
+# (16 B float4 positions and per-bead terms, 4 B indices/counters; neighbour
+# gathers counted once because the sorted copy is read once per launch):
+#   force stage 1: R sorted pos 16 + R bonded/wall force 16 + R normals 16
+#                  + W er 16 + W r+f dt/2+noise 16 + W bucket slot 4
+#                  + count atomic 4 = 88
+#   force stage 2: R sorted pos 16 + R bonded/wall force 16 + R r+f dt/2+noise 16
 #                  + W r 16 + W bucket slot 4 + count atomic 4 = 72
-#   order:         R bucket 4 + R pos 16 + W sorted pos 16 = 36 per bead,
+#   order:         R bucket 4 + R pos 16 + W sorted pos 16 + W bonded/wall force 16
+#                  = 52 per bead (+ W next normals 16 after stage 1),
 #                  + R count 4 + W start 4 per cell
-BYTES_FORCE_PER_BEAD = 72
-BYTES_ORDER_PER_BEAD = 36
+BYTES_FORCE_PER_BEAD = {1: 88, 2: 72}
+BYTES_ORDER_PER_BEAD = {1: 68, 2: 52}
 BYTES_ORDER_PER_CELL = 8
 
 
@@ -187,13 +189,14 @@ def main():
 
     # per-kernel device time (HIP events on the simulation's stream)
     kt = sim.time_kernels(200)
-    force_ms = max(kt[0], kt[2])
+    stage = 1 if kt[0] >= kt[2] else 2
+    force_ms = float(kt[0] if stage == 1 else kt[2])
     kname = "k_force_bal" if shape.get("lanes_per_bead") == 8 else "k_force"
-    dominant = f"{kname}<LJ,1>" if kt[0] >= kt[2] else f"{kname}<LJ,2>"
-    bytes_force = BYTES_FORCE_PER_BEAD * N
+    dominant = f"{kname}<LJ,{stage}>"
+    bytes_force = BYTES_FORCE_PER_BEAD[stage] * N
     achieved = bytes_force / (force_ms * 1e-3) / 1e9
     traffic = read_traffic(args.traffic)
-    order_bytes = BYTES_ORDER_PER_BEAD * N + BYTES_ORDER_PER_CELL * sim.info.n_c
+    order_bytes = BYTES_ORDER_PER_BEAD[stage] * N + BYTES_ORDER_PER_CELL * sim.info.n_c
     sim.synchronize()
 
     cpu = None if args.no_cpu else cpu_baseline(sim, args.cpu_seconds)
