@@ -659,13 +659,22 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
   }
 }
 
+// The per-bead input of the stage update: the step's unit normals (stage 1,
+// zero without noise) or r + f dt/2 + noise (stage 2). Kernels may load it
+// early so that its latency hides behind the pair sum.
+template <int STAGE> __device__ __forceinline__ float4 stage_input(const KParams &p, const KPtrs &q, uint32_t i) {
+  if constexpr (STAGE == 1) return p.sd > 0.0f ? q.noise[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  else if constexpr (STAGE == 2) return q.pstar[i];
+  else return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 // Force cap and the stage update of bead i from its total force (the
 // epilogue of the force kernels): STAGE 0 writes the force, STAGE 1 the Heun
 // predictor and r + f dt/2 + noise, STAGE 2 the corrector; both bin the new
-// position.
+// position. `in` is stage_input<STAGE>.
 template <int STAGE>
-__device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q, uint32_t i, uint32_t w,
-                                               float4 me, float fx, float fy, float fz) {
+__device__ __forceinline__ void stage_update(const KParams &p, const KPtrs &q, uint32_t i, uint32_t w, float4 me,
+                                             float fx, float fy, float fz, float4 in) {
   if (STAGE != 0 && p.fcap2 > 0.0f) {
     const float m2 = fx * fx + fy * fy + fz * fz;
     if (m2 > p.fcap2) {
@@ -676,22 +685,24 @@ __device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q,
   if constexpr (STAGE == 0) {
     q.out0[i] = make_float4(fx, fy, fz, 0.0f);
   } else if constexpr (STAGE == 1) {
-    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    if (p.sd > 0.0f) {
-      const float4 z = q.noise[i];
-      nx = p.sd * z.x; ny = p.sd * z.y; nz = p.sd * z.z;
-    }
+    const float nx = p.sd * in.x, ny = p.sd * in.y, nz = p.sd * in.z;
     const float ex = me.x + fx * p.dt + nx, ey = me.y + fy * p.dt + ny, ez = me.z + fz * p.dt + nz;
     q.out0[i] = make_float4(ex, ey, ez, me.w);
     q.pstar[i] = make_float4(me.x + fx * p.half_dt + nx, me.y + fy * p.half_dt + ny, me.z + fz * p.half_dt + nz,
                              0.0f);
     bin_bead(p, q, i, w, ex, ey, ez);
   } else {
-    const float4 ps = q.pstar[i];
-    const float nx = ps.x + fx * p.half_dt, ny = ps.y + fy * p.half_dt, nz = ps.z + fz * p.half_dt;
+    const float nx = in.x + fx * p.half_dt, ny = in.y + fy * p.half_dt, nz = in.z + fz * p.half_dt;
     q.out0[i] = make_float4(nx, ny, nz, me.w);
     bin_bead(p, q, i, w, nx, ny, nz);
   }
+}
+
+// stage_update with its input loaded here
+template <int STAGE>
+__device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q, uint32_t i, uint32_t w,
+                                               float4 me, float fx, float fy, float fz) {
+  stage_update<STAGE>(p, q, i, w, me, fx, fy, fz, stage_input<STAGE>(p, q, i));
 }
 
 // ---------------------------------------------------------------- balanced force kernel
@@ -905,19 +916,23 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
   const int wbase = (xcd_block() * (BEAD_BLOCK / 64) + wave) * ((64 / C_LANES) * ROUNDS);
   if constexpr (ROUNDS == 1) {
     int t = wbase + gw;
+    float4 me = t < p.N ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (p.group) {
       // A wavefront runs as long as its longest list. Het beads (cutoff 2.5
       // sigma) have about four times the candidates of the others, so the
       // workgroup's slots are reordered: het slots first, then the rest, each
-      // in cell order. Position grp takes the k-th set bit of the het mask.
+      // in cell order. Position grp takes the k-th set bit of the het mask;
+      // the positions move through LDS.
       __shared__ uint32_t s_mask[BEAD_BLOCK / 64];
+      __shared__ float4 s_me[C_BEADS_PER_BLOCK];
       const int t0 = wbase - wave * (64 / C_LANES);
-      const bool het = t < p.N && (wbits(q.fspos[t]) & W_HET) != 0u;
+      const bool het = t < p.N && (wbits(me) & W_HET) != 0u;
       const uint64_t b = __ballot(het && sub == 0);
       uint32_t m8 = 0;
 #pragma unroll
       for (int k = 0; k < 64 / C_LANES; ++k) m8 |= (uint32_t)((b >> (k * C_LANES)) & 1u) << k;
       if (lane == 0) s_mask[wave] = m8;
+      if (sub == 0) s_me[grp] = me;
       __syncthreads();
       uint32_t m = 0;
 #pragma unroll
@@ -935,12 +950,15 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
         }
       }
       t = t0 + pos;
+      me = s_me[pos];
     }
     const bool active = t < p.N;
-    const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const uint32_t w = wbits(me);
     const uint32_t i = w & ID_MASK;
     if constexpr (BATCH) load_system(p, q, active ? (uint32_t)t : 0u);
+    // the stage input is needed only at the end: load it now
+    float4 in = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (EPI && active && sub == 0) in = stage_input<STAGE>(p, q, i);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (EPI && active && sub == 0) {
       if (q.fbond) {
@@ -956,7 +974,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
     if constexpr (!EPI) {
       q.fpair[i] = make_float4(fx, fy, fz, 0.0f);
     } else {
-      stage_epilogue<STAGE>(p, q, i, w, me, fx, fy, fz);
+      stage_update<STAGE>(p, q, i, w, me, fx, fy, fz, in);
     }
   } else {
     float kx = 0.0f, ky = 0.0f, kz = 0.0f;
