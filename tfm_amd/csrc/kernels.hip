@@ -871,13 +871,25 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
   if (listed) {
+    // four gathers in flight per lane; a short tail is padded with the bead
+    // itself, whose term is exactly zero (|i - j| <= 1 is excluded)
     uint32_t n = sub;
-    for (; n + C_LANES < total; n += 2 * C_LANES) {
-      const float4 a = q.fspos[list[n]], b = q.fspos[list[n + C_LANES]];
+    for (; n + 3 * C_LANES < total; n += 4 * C_LANES) {
+      const float4 a = q.fspos[list[n]], b = q.fspos[list[n + C_LANES]], c = q.fspos[list[n + 2 * C_LANES]],
+                   d = q.fspos[list[n + 3 * C_LANES]];
       pair_full<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
       pair_full<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
+      pair_full<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
+      pair_full<MODE>(p, rx, ry, rz, w, d, fx, fy, fz);
     }
-    if (n < total) pair_full<MODE>(p, rx, ry, rz, w, q.fspos[list[n]], fx, fy, fz);
+    if (n < total) {
+      const float4 a = q.fspos[list[n]];
+      const float4 b = n + C_LANES < total ? q.fspos[list[n + C_LANES]] : me;
+      const float4 c = n + 2 * C_LANES < total ? q.fspos[list[n + 2 * C_LANES]] : me;
+      pair_full<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
+      pair_full<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
+      pair_full<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
+    }
   } else {
     // crowded neighbourhood: each lane scans its own rows
 #pragma unroll
