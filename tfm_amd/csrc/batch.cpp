@@ -27,8 +27,8 @@ thread_local std::string g_batch_create_error;
 // Host state of one uniform grid over the concatenated cell range.
 struct BatchGrid {
   GridGeom g{}; // n_c, cap, nblk of the whole range
-  DevBuf<uint32_t> count, bs0, bs1, bucket, cstart;
-  DevBuf<float4> spos;
+  DevBuf<uint32_t> count, bs0, bs1, cstart;
+  DevBuf<float4> bucket, spos; // bucket: members' positions, index in the w bits
   int clean = 0;
   bool valid = false;
   uint32_t *bs(int k) { return k ? bs1.p : bs0.p; }
@@ -47,8 +47,9 @@ struct BatchGrid {
     valid = false;
   }
   void free_() {
-    DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &bucket, &cstart};
+    DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &cstart};
     for (auto *b : u) b->free_();
+    bucket.free_();
     spos.free_();
   }
 };
@@ -153,7 +154,7 @@ struct ccp_batch {
     for (;;) {
       launch_bin_batch(stream, dsys.p, dslot.p, (int)n_sys, (int)N, r.p, grid.count.p, grid.bs(grid.clean),
                        grid.bucket.p, flags.p);
-      OrderJob j{grid.g, grid.count.p, grid.bs(grid.clean), grid.bs(grid.clean ^ 1), grid.bucket.p, r.p,
+      OrderJob j{grid.g, grid.count.p, grid.bs(grid.clean), grid.bs(grid.clean ^ 1), grid.bucket.p,
                  grid.spos.p, nullptr, grid.cstart.p};
       launch_order(stream, j, nullptr, flags.p, nullptr);
       hip_check(hipGetLastError());
@@ -214,7 +215,7 @@ struct ccp_batch {
     q.sys_slot = dslot.p;
     q.n_sys = (int)n_sys;
     launch_force(stream, MODE_LJ, stage, kp, q);
-    OrderJob j{grid.g, grid.count.p, grid.bs(grid.clean), grid.bs(grid.clean ^ 1), grid.bucket.p, q.out0,
+    OrderJob j{grid.g, grid.count.p, grid.bs(grid.clean), grid.bs(grid.clean ^ 1), grid.bucket.p,
                grid.spos.p, nullptr, grid.cstart.p};
     // with the order pass: the next step's normals after the first stage,
     // the bonded terms of q.out0 after both

@@ -342,7 +342,7 @@ void ccp_sim::check_flags() {
 void ccp_sim::build_grid(HostGrid &G, const GridGeom &geom, const float4 *pos, int n_obj, bool het_only,
                          float4 *out_spos, uint32_t *out_cstart, uint32_t *out_soi) {
   launch_bin(stream, kp, geom, n_obj, pos, het_only, G.count.p, G.bs(G.clean), G.bucket.p, flags.p);
-  OrderJob j{geom, G.count.p, G.bs(G.clean), G.bs(G.clean ^ 1), G.bucket.p, pos, out_spos, out_soi, out_cstart};
+  OrderJob j{geom, G.count.p, G.bs(G.clean), G.bs(G.clean ^ 1), G.bucket.p, out_spos, out_soi, out_cstart};
   launch_order(stream, j, nullptr, flags.p, nullptr);
   hip_check(hipGetLastError());
   G.clean ^= 1;
@@ -430,9 +430,9 @@ void ccp_sim::launch_stage(int mode, int stage, hipEvent_t mid) {
   q.fbond = bond_pass() ? fbond.p : nullptr;
   launch_force(stream, mode, stage, k, q);
   if (mid) hip_check(hipEventRecord(mid, stream));
-  OrderJob jf{gf.g, gf.count.p, gf.bs(gf.clean), gf.bs(gf.clean ^ 1), gf.bucket.p, q.out0, gf.spos.p, nullptr,
+  OrderJob jf{gf.g, gf.count.p, gf.bs(gf.clean), gf.bs(gf.clean ^ 1), gf.bucket.p, gf.spos.p, nullptr,
               gf.cstart.p};
-  OrderJob jh{gh.g, gh.count.p, gh.bs(gh.clean), gh.bs(gh.clean ^ 1), gh.bucket.p, q.out0, gh.spos.p, nullptr,
+  OrderJob jh{gh.g, gh.count.p, gh.bs(gh.clean), gh.bs(gh.clean ^ 1), gh.bucket.p, gh.spos.p, nullptr,
               gh.cstart.p};
   // With the order pass: after the first stage the counter still holds this
   // step, so draw the next one's normals; the bonded terms of q.out0.

@@ -141,24 +141,25 @@ __device__ __forceinline__ int cell_coord(float x, const GridGeom &g, bool &outs
   return c;
 }
 
-__device__ __forceinline__ void bin_into(const GridGeom &g, uint32_t *count, uint32_t *blocksum, uint32_t *bucket,
-                                         uint32_t *flags, uint32_t i, float x, float y, float z) {
+// The bucket entry is the member's position with its index in the w bits.
+__device__ __forceinline__ void bin_into(const GridGeom &g, uint32_t *count, uint32_t *blocksum, float4 *bucket,
+                                         uint32_t *flags, float4 v) {
   bool outside = false;
-  const int gx = cell_coord(x, g, outside), gy = cell_coord(y, g, outside), gz = cell_coord(z, g, outside);
+  const int gx = cell_coord(v.x, g, outside), gy = cell_coord(v.y, g, outside), gz = cell_coord(v.z, g, outside);
   const uint32_t key = (uint32_t)(g.coff + (gz * g.cps + gy) * g.cps + gx);
   // beads of one wavefront mostly sit in distinct cells: plain atomics
   const uint32_t slot = atomicAdd(count + key, 1u);
-  if (slot < (uint32_t)g.cap) bucket[(size_t)key * g.cap + slot] = i;
+  if (slot < (uint32_t)g.cap) bucket[(size_t)key * g.cap + slot] = v;
   agg_add_noret(blocksum, key / SCAN_BLOCK);
   if (outside) atomicAdd(flags + FLAG_OUTSIDE, 1u);
 }
 
 // Bin a new bead position into the next fine grid and, for het beads, the
 // next het grid (the het-grid atomics run on the het lanes only).
-__device__ __forceinline__ void bin_bead(const KParams &p, const KPtrs &q, uint32_t i, uint32_t w, float x, float y,
-                                         float z) {
-  bin_into(p.gf, q.fcount, q.fblocksum, q.fbucket, q.flags, i, x, y, z);
-  if (p.gh.n_c > 0 && (w & W_HET)) bin_into(p.gh, q.hcount, q.hblocksum, q.hbucket, q.flags, i, x, y, z);
+__device__ __forceinline__ void bin_bead(const KParams &p, const KPtrs &q, uint32_t w, float x, float y, float z) {
+  const float4 v = make_float4(x, y, z, __uint_as_float(w)); // w carries the bead index
+  bin_into(p.gf, q.fcount, q.fblocksum, q.fbucket, q.flags, v);
+  if (p.gh.n_c > 0 && (w & W_HET)) bin_into(p.gh, q.hcount, q.hblocksum, q.hbucket, q.flags, v);
   if (!(isfinite(x) && isfinite(y) && isfinite(z))) atomicAdd(q.flags + FLAG_NONFINITE, 1u);
 }
 
@@ -650,12 +651,12 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
     const float ex = rx + fx * p.dt + nx, ey = ry + fy * p.dt + ny, ez = rz + fz * p.dt + nz;
     q.out0[i] = make_float4(ex, ey, ez, me.w);
     q.pstar[i] = make_float4(rx + fx * p.half_dt + nx, ry + fy * p.half_dt + ny, rz + fz * p.half_dt + nz, 0.0f);
-    bin_bead(p, q, i, w, ex, ey, ez);
+    bin_bead(p, q, w, ex, ey, ez);
   } else {
     const float4 ps = q.pstar[i];
     const float nx = ps.x + fx * p.half_dt, ny = ps.y + fy * p.half_dt, nz = ps.z + fz * p.half_dt;
     q.out0[i] = make_float4(nx, ny, nz, me.w);
-    bin_bead(p, q, i, w, nx, ny, nz);
+    bin_bead(p, q, w, nx, ny, nz);
   }
 }
 
@@ -690,11 +691,11 @@ __device__ __forceinline__ void stage_update(const KParams &p, const KPtrs &q, u
     q.out0[i] = make_float4(ex, ey, ez, me.w);
     q.pstar[i] = make_float4(me.x + fx * p.half_dt + nx, me.y + fy * p.half_dt + ny, me.z + fz * p.half_dt + nz,
                              0.0f);
-    bin_bead(p, q, i, w, ex, ey, ez);
+    bin_bead(p, q, w, ex, ey, ez);
   } else {
     const float nx = in.x + fx * p.half_dt, ny = in.y + fy * p.half_dt, nz = in.z + fz * p.half_dt;
     q.out0[i] = make_float4(nx, ny, nz, me.w);
-    bin_bead(p, q, i, w, nx, ny, nz);
+    bin_bead(p, q, w, nx, ny, nz);
   }
 }
 
@@ -1040,38 +1041,40 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_integrate(KParams p, KPtrs q) {
 
 // Binning of an index-ordered array (initial builds, LBS grid, export grid).
 __global__ __launch_bounds__(BEAD_BLOCK) void k_bin(GridGeom g, int n_obj, const float4 *pos, int het_only,
-                                                    uint32_t *count, uint32_t *blocksum, uint32_t *bucket,
+                                                    uint32_t *count, uint32_t *blocksum, float4 *bucket,
                                                     uint32_t *flags) {
   const int i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
   if (i >= n_obj) return;
   const float4 v = pos[i];
   if (het_only && !(wbits(v) & W_HET)) return;
-  bin_into(g, count, blocksum, bucket, flags, (uint32_t)i, v.x, v.y, v.z);
+  bin_into(g, count, blocksum, bucket, flags, v);
 }
 
 // Binning of the index-ordered beads of a batch into their systems' grids
 // (bead i belongs to the system whose slot range holds i).
 __global__ __launch_bounds__(BEAD_BLOCK) void k_bin_batch(const SysParams *sys, const uint32_t *sys_slot, int n_sys,
                                                           int n, const float4 *pos, uint32_t *count,
-                                                          uint32_t *blocksum, uint32_t *bucket, uint32_t *flags) {
+                                                          uint32_t *blocksum, float4 *bucket, uint32_t *flags) {
   const int i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
   if (i >= n) return;
   const float4 v = pos[i];
   const SysParams &S = sys[system_of(sys_slot, n_sys, (uint32_t)i)];
-  bin_into(S.g, count, blocksum, bucket, flags, (uint32_t)i, v.x, v.y, v.z);
+  bin_into(S.g, count, blocksum, bucket, flags, v);
 }
 
-// Per-cell scan + deterministic order + gather of one or two grids
-// (replaces ccp's calc_indexes/SortPairs/set_cells_empty/find_cells_limits
-// sequence, sugrid.cu:125-133). The block's bucket entries are staged in
-// LDS; each entry's rank within its cell (number of smaller bead indices)
-// is then found in parallel.
-constexpr int ORDER_LDS = 4096;
+// Per-cell scan + deterministic order of one or two grids (replaces ccp's
+// calc_indexes/SortPairs/set_cells_empty/find_cells_limits sequence,
+// sugrid.cu:125-133). The bucket holds each member's position with its index
+// in the w bits. The block's entries are staged in LDS; each entry's rank
+// within its cell (number of smaller indices) is then found in parallel and
+// the entry written to its sorted slot.
+constexpr int ORDER_LDS = 1024;
 
 __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, uint32_t *flags,
                                                       uint64_t *step_inc, NoiseJob nj, KParams bp, BondJob bj) {
   __shared__ uint32_t red[SCAN_BLOCK / 64], wsum[SCAN_BLOCK / 64], msum[SCAN_BLOCK / 64];
   __shared__ uint32_t s_id[ORDER_LDS];
+  __shared__ float4 s_pos[ORDER_LDS];
   __shared__ uint8_t s_cell[ORDER_LDS];
   __shared__ uint32_t s_loff[SCAN_BLOCK + 1];
   __shared__ uint32_t s_start[SCAN_BLOCK];
@@ -1129,13 +1132,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
     J.bs_clear[b] = 0u;
     if (b == 0 && !second && step_inc) *step_inc += 1ull;
   }
-  const uint32_t *bk = J.bucket + (size_t)c * cap;
+  const float4 *bk = J.bucket + (size_t)c * cap;
   if (mtotal <= (uint32_t)ORDER_LDS) {
     s_loff[tid] = moff;
     s_start[tid] = start;
     if (tid == SCAN_BLOCK - 1) s_loff[SCAN_BLOCK] = moff + m;
     for (uint32_t k = 0; k < m; ++k) {
-      s_id[moff + k] = bk[k];
+      const float4 v = bk[k];
+      s_pos[moff + k] = v;
+      s_id[moff + k] = __float_as_uint(v.w) & ID_MASK;
       s_cell[moff + k] = (uint8_t)tid;
     }
     __syncthreads();
@@ -1145,16 +1150,17 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
       uint32_t rank = 0;
       for (uint32_t s = b0; s < b1; ++s) rank += (s_id[s] < id) ? 1u : 0u;
       const uint32_t dst = s_start[l] + rank;
-      J.spos[dst] = J.pos[id];
+      J.spos[dst] = s_pos[e];
       if (J.soi) J.soi[dst] = id;
     }
   } else {
     // dense block (raw initial walks): rank each cell's entries in place
     for (uint32_t k = 0; k < m; ++k) {
-      const uint32_t e = bk[k];
+      const float4 v = bk[k];
+      const uint32_t e = __float_as_uint(v.w) & ID_MASK;
       uint32_t rank = 0;
-      for (uint32_t s = 0; s < m; ++s) rank += (bk[s] < e) ? 1u : 0u;
-      J.spos[start + rank] = J.pos[e];
+      for (uint32_t s = 0; s < m; ++s) rank += (__float_as_uint(bk[s].w) & ID_MASK) < e ? 1u : 0u;
+      J.spos[start + rank] = v;
       if (J.soi) J.soi[start + rank] = e;
     }
   }
@@ -1384,7 +1390,7 @@ inline unsigned blocks_for(int n, int bs) { return (unsigned)((n + bs - 1) / bs)
 } // namespace
 
 void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, const float4 *pos, bool het_only,
-                uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags) {
+                uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags) {
   (void)p;
   if (n_obj <= 0) return;
   k_bin<<<blocks_for(n_obj, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(g, n_obj, pos, het_only ? 1 : 0, count, blocksum,
@@ -1534,7 +1540,7 @@ void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, f
 }
 
 void launch_bin_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n,
-                      const float4 *pos, uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags) {
+                      const float4 *pos, uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags) {
   if (n <= 0) return;
   k_bin_batch<<<blocks_for(n, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(sys, sys_slot, n_sys, n, pos, count, blocksum, bucket,
                                                               flags);

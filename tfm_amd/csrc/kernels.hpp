@@ -56,7 +56,7 @@ struct SysParams {
 struct GridBufs {
   uint32_t *count;
   uint32_t *bs[2];   // per-scan-block totals, double buffered
-  uint32_t *bucket;  // n_c * cap unsorted members
+  float4 *bucket;    // n_c * cap unsorted members (position, index in the w bits)
   uint32_t *cstart;  // n_c + 1 starts into spos
   float4 *spos;      // positions in cell order
   uint32_t *soi;     // bead index in cell order (optional)
@@ -106,8 +106,10 @@ struct KPtrs {
   const float4 *pos;       // bead positions of the evaluated stage, by bead index
   float4 *out0;            // stage 1: er, stage 2: new r, stage 0: force
   float4 *pstar;           // r + f dt/2 + noise (written by stage 1, read by stage 2)
-  uint32_t *fcount, *fblocksum, *fbucket; // binning of the output into the next fine grid
-  uint32_t *hcount, *hblocksum, *hbucket; // ... and into the next het grid
+  uint32_t *fcount, *fblocksum; // binning of the output into the next fine grid
+  float4 *fbucket;
+  uint32_t *hcount, *hblocksum; // ... and into the next het grid
+  float4 *hbucket;
   uint32_t *flags;         // FLAG_COUNT error counters
   const uint64_t *step;    // global step counter (noise counter)
   float4 *fpair;           // pair forces by bead index (balanced kernel with a separate integration pass)
@@ -125,8 +127,7 @@ struct OrderJob {
   uint32_t *count;
   const uint32_t *bs_read;
   uint32_t *bs_clear;
-  const uint32_t *bucket;
-  const float4 *pos;
+  const float4 *bucket; // members' positions, index in the w bits
   float4 *spos;
   uint32_t *soi;
   uint32_t *cstart;
@@ -161,7 +162,7 @@ struct BondJob {
 
 // Launchers (kernels.hip). All are asynchronous on `s`.
 void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, const float4 *pos, bool het_only,
-                uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags);
+                uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags);
 // The order pass also fills the next force pass's per-bead terms when given:
 // normals (noise) and bonded + wall forces (bp, bond).
 void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc,
@@ -182,7 +183,7 @@ void launch_sd_cp(hipStream_t s, const float4 *pos, uint32_t N, uint32_t lsdcp, 
                   float *sd, float *cp, double *sum_sd, double *sum_cp);
 void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm, double *sum);
 void launch_bin_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n,
-                      const float4 *pos, uint32_t *count, uint32_t *blocksum, uint32_t *bucket, uint32_t *flags);
+                      const float4 *pos, uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags);
 void launch_close_sums(hipStream_t s, double *sum, double *m1, double *m2, uint32_t n, double n_dp);
 
 } // namespace ccpmi

@@ -138,8 +138,8 @@ struct ccp_sim {
   // one uniform grid: binning buffers + ordered cell list
   struct HostGrid {
     GridGeom g{};
-    DevBuf<uint32_t> count, bs0, bs1, bucket, cstart, soi;
-    DevBuf<float4> spos;
+    DevBuf<uint32_t> count, bs0, bs1, cstart, soi;
+    DevBuf<float4> bucket, spos; // bucket: members' positions, index in the w bits
     int clean = 0;       // which per-block total buffer is zero
     bool valid = false;  // holds the cell list of the current positions
     bool allocated = false;
@@ -156,8 +156,9 @@ struct ccp_sim {
       allocated = true;
     }
     void free_() {
-      DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &bucket, &cstart, &soi};
+      DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &cstart, &soi};
       for (auto *b : u) b->free_();
+      bucket.free_();
       spos.free_();
       allocated = false;
     }
