@@ -803,10 +803,12 @@ constexpr uint64_t RING2_WORDS[4] = {ring2_word(0), ring2_word(1), ring2_word(2)
 // Pair (and LBS) forces of the bead `me` over its trimmed candidate list,
 // spread over the bead's eight lanes (sub = 0..7) and added to f; on return
 // every lane of the group holds the bead's total. `list` is the group's LDS
-// list; the caller's lanes of one wavefront run this together.
+// list, `self` the bead's own cell-order slot; the caller's lanes of one
+// wavefront run this together.
 template <int MODE, bool BATCH, int R>
 __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, uint32_t *list, bool active,
-                                             int sub, int lane, float4 me, float &fx, float &fy, float &fz) {
+                                             uint32_t self, int sub, int lane, float4 me, float &fx, float &fy,
+                                             float &fz) {
   constexpr int W = 2 * R + 1, NROW = W * W, RPL = (NROW + C_LANES - 1) / C_LANES;
   const uint32_t w = wbits(me);
   const float rx = me.x, ry = me.y, rz = me.z;
@@ -872,8 +874,8 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
   if (listed) {
-    // four gathers in flight per lane; a short tail is padded with the bead
-    // itself, whose term is exactly zero (|i - j| <= 1 is excluded)
+    // four gathers in flight per lane; a short tail is padded with the bead's
+    // own slot, whose term is exactly zero (|i - j| <= 1 is excluded)
     uint32_t n = sub;
     for (; n + 3 * C_LANES < total; n += 4 * C_LANES) {
       const float4 a = q.fspos[list[n]], b = q.fspos[list[n + C_LANES]], c = q.fspos[list[n + 2 * C_LANES]],
@@ -884,9 +886,9 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
       pair_full<MODE>(p, rx, ry, rz, w, d, fx, fy, fz);
     }
     if (n < total) {
-      const float4 a = q.fspos[list[n]];
-      const float4 b = n + C_LANES < total ? q.fspos[list[n + C_LANES]] : me;
-      const float4 c = n + 2 * C_LANES < total ? q.fspos[list[n + 2 * C_LANES]] : me;
+      const uint32_t jb = n + C_LANES < total ? list[n + C_LANES] : self;
+      const uint32_t jc = n + 2 * C_LANES < total ? list[n + 2 * C_LANES] : self;
+      const float4 a = q.fspos[list[n]], b = q.fspos[jb], c = q.fspos[jc];
       pair_full<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
       pair_full<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
       pair_full<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
@@ -982,7 +984,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
         wall_force<MODE>(p, me.x, me.y, me.z, w, fx, fy, fz);
       }
     }
-    bal_pair_sum<MODE, BATCH, R>(p, q, s_list[grp], active, sub, lane, me, fx, fy, fz);
+    bal_pair_sum<MODE, BATCH, R>(p, q, s_list[grp], active, (uint32_t)t, sub, lane, me, fx, fy, fz);
     if (!active || sub != 0) return;
     if constexpr (!EPI) {
       q.fpair[i] = make_float4(fx, fy, fz, 0.0f);
@@ -997,7 +999,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
       const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if constexpr (BATCH) load_system(p, q, active ? (uint32_t)t : 0u);
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-      bal_pair_sum<MODE, BATCH, R>(p, q, s_list[grp], active, sub, lane, me, fx, fy, fz);
+      bal_pair_sum<MODE, BATCH, R>(p, q, s_list[grp], active, (uint32_t)t, sub, lane, me, fx, fy, fz);
       if (sub == rd) {
         kx = fx;
         ky = fy;
