@@ -231,7 +231,6 @@ void ccp_sim::fill_kparams() {
   k.lco2 = LCO * LCO;
   k.inv_lco2 = 1.0f / (LCO * LCO);
   k.eps_wca = 1.0f;
-  k.sig2 = 1.0f;
   k.rm2 = (float)std::pow(2.0, 1.0 / 3.0);
   k.eps_het = eps_het;
   k.rc_lj2 = 2.5f * 2.5f;

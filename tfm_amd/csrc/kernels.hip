@@ -213,7 +213,7 @@ __device__ __forceinline__ void pair_short(const KParams &p, float rx, float ry,
     fac = (nb && d <= p.rco) ? fac : 0.0f;
   } else {
     const float u = fast_rcp(d2);
-    const float s2 = p.sig2 * u;
+    const float s2 = u;
     const float s6 = s2 * s2 * s2;
     fac = p.eps_wca * 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
     fac = (nb && d2 < p.rm2) ? fac : 0.0f;
@@ -240,7 +240,7 @@ __device__ __forceinline__ void pair_tail(const KParams &p, float rx, float ry, 
     fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
     fac = (nb && d2 > p.rco2 && d2 <= p.aco2) ? fac : 0.0f;
   } else {
-    const float s2 = p.sig2 * u;
+    const float s2 = u;
     const float s6 = s2 * s2 * s2;
     fac = p.eps_het * 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
     fac = (nb && d2 >= p.rm2 && d2 < p.rc_lj2) ? fac : 0.0f;
@@ -301,7 +301,7 @@ __device__ __forceinline__ void bonded_force(const KParams &p, const float4 *__r
       const float d2 = v2x * v2x + v2y * v2y + v2z * v2z;
       float c = p.fene_k / fmaxf(1.0f - d2 / p.fene_r02, 0.01f);
       if (d2 < p.rm2) {
-        const float u = fast_rcp(d2), s2 = p.sig2 * u, s6 = s2 * s2 * s2;
+        const float u = fast_rcp(d2), s2 = u, s6 = s2 * s2 * s2;
         c -= p.eps_wca * 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
       }
       fx += c * v2x; fy += c * v2y; fz += c * v2z;
@@ -310,7 +310,7 @@ __device__ __forceinline__ void bonded_force(const KParams &p, const float4 *__r
       const float d2 = v1x * v1x + v1y * v1y + v1z * v1z;
       float c = -p.fene_k / fmaxf(1.0f - d2 / p.fene_r02, 0.01f);
       if (d2 < p.rm2) {
-        const float u = fast_rcp(d2), s2 = p.sig2 * u, s6 = s2 * s2 * s2;
+        const float u = fast_rcp(d2), s2 = u, s6 = s2 * s2 * s2;
         c += p.eps_wca * 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
       }
       fx += c * v1x; fy += c * v1y; fz += c * v1z;
@@ -377,7 +377,7 @@ __device__ __forceinline__ void wall_force(const KParams &p, float rx, float ry,
     fac = u * u * (18.0f + u * (96.0f * u - 96.0f));
   } else {
     const float u = fast_rcp(dw2);
-    const float s2 = p.sig2 * u, s6 = s2 * s2 * s2;
+    const float s2 = u, s6 = s2 * s2 * s2;
     const float base = 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
     if (dw2 < p.rm2) fac = p.eps_wall * base;
     else if ((w & W_LAD) && dw2 < p.rc_lam2) fac = p.eps_lam * base;
@@ -400,7 +400,9 @@ __device__ __forceinline__ void pair_full(const KParams &p, float rx, float ry, 
   const uint32_t i = wi & ID_MASK, j = wj & ID_MASK;
   const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
   const float d2 = dx * dx + dy * dy + dz * dz;
-  const bool nb = (j > i + 1u) || (i > j + 1u);
+  // |i - j| > 1 as one unsigned compare: j - (i - 1) wraps past 2 unless
+  // j is i - 1, i or i + 1
+  const bool nb = j - (i - 1u) > 2u;
   const bool hhi = (wi & wj & W_HET) != 0u;
   float fac;
   if constexpr (MODE == MODE_CCP) {
@@ -412,13 +414,14 @@ __device__ __forceinline__ void pair_full(const KParams &p, float rx, float ry, 
     fac = 128.0f * (3.0f * p.rco - 3.0f * d);
     fac = (nb && d <= p.rco) ? fac : 0.0f;
   } else {
+    // sigma = 1: 24 eps (2 r^-12 - r^-6) / r^2, WCA within 2^(1/6) for
+    // every pair, the LJ tail beyond it for het-het pairs
     const float u = fast_rcp(d2);
-    const float s2 = p.sig2 * u;
-    const float s6 = s2 * s2 * s2;
-    const float base = 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
-    const bool wca = nb && d2 < p.rm2;
-    const bool tail = nb && hhi && !wca && d2 < p.rc_lj2;
-    fac = wca ? p.eps_wca * base : (tail ? p.eps_het * base : 0.0f);
+    const float s6 = u * u * u;
+    const float base = s6 * fmaf(48.0f, s6, -24.0f) * u;
+    const bool wca = d2 < p.rm2;
+    const bool on = nb && (wca || (hhi && d2 < p.rc_lj2));
+    fac = on ? (wca ? p.eps_wca : p.eps_het) * base : 0.0f;
   }
   fx += fac * dx;
   fy += fac * dy;

@@ -81,8 +81,8 @@ struct KParams {
   float e_p, rco, rco2, aco2;
   // LBS binding
   float lfac, lco2, inv_lco2;
-  // WCA/LJ set
-  float eps_wca, sig2, rm2, eps_het, rc_lj2, eps_wall, eps_lam, rc_lam2;
+  // WCA/LJ set in units of sigma (sigma = 1)
+  float eps_wca, rm2, eps_het, rc_lj2, eps_wall, eps_lam, rc_lam2;
   // squared candidate-list radii of the balanced kernel: cutoff + 1e-3 sigma
   // for het beads (the het-het cutoff) and for the others (the short one), in
   // the simulation's pair set (the relaxation set uses rco for every bead)
