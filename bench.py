@@ -31,6 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# vector-ALU issue: 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
+# (no packed FP32), in G wave-instructions/s
+VALU_PEAK_GWIPS = 1024 * 2.4 / 4
 N_CHAINS, CHAIN_LEN = 8, 16384
 N = N_CHAINS * CHAIN_LEN
 RCO = float(np.float32(1.154701))
@@ -80,6 +83,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_force_lj_n131072_traffic.txt"))
+    ap.add_argument("--sq", default=os.path.join(ROOT, "profiles", "k_force_bal_lj_n131072_sq.txt"))
     ap.add_argument("--param", action="append", default=[], metavar="KEY=VALUE",
                     help="override a simulation parameter (e.g. fine_cell_size=1.25)")
     ap.add_argument("--no-tune", action="store_true", help="skip the kernel-shape autotuner")
@@ -116,6 +120,33 @@ def read_traffic(path):
     return None
 
 
+def read_valu(path):
+    """Vector-ALU issue of the dominant kernel from a committed rocprofv3 SQ
+    summary: (wave-instructions per launch, mean duration of those dispatches
+    in us), or None."""
+    vals = {}
+    try:
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 2 and parts[0] in ("valu_insts_per_launch", "valu_dispatch_us"):
+                vals[parts[0]] = float(parts[1])
+    except OSError:
+        return None
+    if len(vals) != 2:
+        return None
+    return vals["valu_insts_per_launch"], vals["valu_dispatch_us"]
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(sim, budget_s):
     """FP64 oracle on the same state, as many steps as fit in budget_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -137,8 +168,8 @@ def cpu_baseline(sim, budget_s):
             break
     dt = time.perf_counter() - t0
     return {"value": N * steps / dt, "unit": "bead-steps/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 scalar oracle (OpenMP, {cores} threads) on the same N={N} state, "
-                      f"{steps} steps in {dt:.1f} s"}
+            "sample": f"FP64 scalar oracle (OpenMP, {cores} threads of {os.cpu_count()} on {cpu_model()}) "
+                      f"on the same N={N} state, {steps} steps in {dt:.1f} s"}
 
 
 def main():
@@ -196,6 +227,7 @@ def main():
     bytes_force = BYTES_FORCE_PER_BEAD[stage] * N
     achieved = bytes_force / (force_ms * 1e-3) / 1e9
     traffic = read_traffic(args.traffic)
+    valu = read_valu(args.sq)
     order_bytes = BYTES_ORDER_PER_BEAD[stage] * N + BYTES_ORDER_PER_CELL * sim.info.n_c
     sim.synchronize()
 
@@ -232,6 +264,12 @@ def main():
             "kernel_ms": {"force_stage1": float(kt[0]), "order_stage1": float(kt[1]),
                           "force_stage2": float(kt[2]), "order_stage2": float(kt[3]), "step": float(kt[4])},
             "bytes_per_launch": {"force": bytes_force, "order": order_bytes},
+            # secondary bound: vector-ALU issue of the same kernel (no packed FP32), from the
+            # committed SQ counter run (its instruction count over its own dispatch time)
+            "valu": None if valu is None else {
+                "achieved": valu[0] / (valu[1] * 1e-6) / 1e9, "peak": VALU_PEAK_GWIPS,
+                "unit": "G wave-instructions/s", "frac": valu[0] / (valu[1] * 1e-6) / 1e9 / VALU_PEAK_GWIPS,
+                "insts_per_launch": valu[0]},
         },
         "cpu_baseline": cpu,
     }
