@@ -133,3 +133,18 @@ def test_batch_autotuner_picks_a_measured_shape():
     best, table = autotune.tune_batch(systems, dict(timestep=0.001), steps=50, warmup=10)
     assert len(table) == 6 and all(np.isfinite(t) and t > 0 for _, t in table)
     assert best == min(table, key=lambda e: e[1])[0]
+
+
+def test_ensemble_runner_small():
+    # the ensemble runner end to end on a few uneven systems: finite state,
+    # symmetric maps, one sample per interval, densities that integrate
+    step("ensemble runner")
+    from tfm_amd import ensemble
+
+    out = ensemble.run(["--systems", "6", "--min-beads", "256", "--max-beads", "1024", "--steps", "2000",
+                        "--interval", "500", "--no-tune"])
+    assert out["systems"] == 6 and 6 * 256 <= out["beads"] <= 6 * 1024
+    assert out["finite_state"] and out["maps_symmetric"]
+    assert out["contact_samples"] == 4
+    assert out["contacts_per_system_mean"] > 0 and out["bead_steps_per_s"] > 0
+    assert len(out["radial_density_mean"]) == 32 and min(out["radial_density_mean"]) >= 0

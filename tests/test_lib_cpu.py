@@ -92,3 +92,17 @@ def test_sequence_pattern_is_host_only_and_deterministic():
     assert np.array_equal(a, sequence_pattern({**p, "replica": 3}, 4096))
     with pytest.raises(tfm_amd.CcpError):
         sequence_pattern(p, 100)
+
+
+def test_ensemble_draws():
+    # chain lengths log-uniform in range, parameters in their intervals, seeds
+    # consecutive, the same draw for the same seed
+    from tfm_amd.ensemble import draw_systems
+
+    a = draw_systems(64, 512, 16384, 4000)
+    assert a == draw_systems(64, 512, 16384, 4000)
+    n = np.array([s["n_beads"] for s in a])
+    assert n.min() >= 512 and n.max() <= 16384 and np.median(n) < (512 + 16384) / 2
+    for s, i in zip(a, range(4000, 4064)):
+        assert s["seed"] == i
+        assert 0.1 <= s["volume_fraction"] <= 0.35 and 0.2 <= s["eps_het"] <= 1.0 and 0.2 <= s["eps_lam"] <= 1.5
