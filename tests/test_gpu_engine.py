@@ -194,6 +194,19 @@ def test_ccp_perform_cli(tmp_path):
     assert h[14] == 4  # frame index continues from the checkpoint
 
 
+def test_kernel_shape_conflicts_are_rejected():
+    # shapes the kernels cannot honour fail at creation with a parameter error
+    step("shape conflicts")
+    bad = [({"lanes_per_bead": 8, "fine_cell_size": 1.2}, "excludes fine_cell_size"),
+           ({"lanes_per_bead": 4}, "lanes_per_bead out of range"),
+           ({"cell_reach": 2, "cell_size": 2.0, "fine_cell_size": 1.2}, "needs cell_reach 1"),
+           ({"bal_rounds": 3, "lanes_per_bead": 8}, "bal_rounds out of range"),
+           ({"bal_rounds": 2, "lanes_per_bead": 8, "integrate_pass": 1}, "excludes integrate_pass")]
+    for kw, msg in bad:
+        with pytest.raises(tfm_amd.CcpError, match=msg):
+            small_sim(**kw)
+
+
 def test_bead_outside_grid_is_reported():
     step("error flag")
     a = small_sim()

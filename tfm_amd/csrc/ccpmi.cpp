@@ -136,6 +136,8 @@ void ccp_sim::configure(const ParMap &par) {
   if (!(fcap >= 0.0f)) throw Error(CCP_ERR_PARAM, "force_cap out of range");
   lanes = par.get<uint32_t>("lanes_per_bead", 2 * reach + 1);
   if (lanes != 1 && lanes != 2 * reach + 1 && lanes != 8) throw Error(CCP_ERR_PARAM, "lanes_per_bead out of range");
+  // the balanced kernel scans one grid: the split pair sum has no 8-lane form
+  if (lanes == 8 && split) throw Error(CCP_ERR_PARAM, "lanes_per_bead 8 excludes fine_cell_size");
   // with 8 lanes per bead: bonds, wall and the stage in a second, one-lane-per-bead
   // pass (off by default: measured 15 % slower at N = 131072)
   integrate_pass = par.get<uint32_t>("integrate_pass", 0u) != 0;
