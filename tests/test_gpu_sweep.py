@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 def test_reduced_sweep_learns_the_radial_density():
     step("reduced sweep: 320 sets x 3000 steps")
-    out = sweep.run(["--train", "320", "--warmup", "500", "--steps", "2500", "--interval", "100", "--epochs", "120",
+    out = sweep.run(["--no-tune", "--train", "320", "--warmup", "500", "--steps", "2500", "--interval", "100", "--epochs", "120",
                      "--candidates", "512", "--top", "32", "--single", "1", "--batch", "160"])
     err = out["contact_map_error"]
     step(f"  radial density MAE {err['radial_density_mae']}, map error {err['surrogate_validation']:.3f} "

@@ -24,6 +24,7 @@ import numpy as np
 import torch
 
 from . import surrogate as sg
+from .autotune import tune_batch
 from .ccp import Batch, Simulation, sequence_pattern
 
 RCO = float(np.float32(1.154701))
@@ -208,14 +209,23 @@ def run(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--replicas", type=int, default=1, help="independent runs averaged per parameter set")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the batch autotuner (8 lanes per bead, reach 2, 1.25 sigma cells)")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    shape = {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.25}
     rng = np.random.default_rng(a.seed)
     t_start = time.perf_counter()
 
-    # data from the batch engine
+    # data from the batch engine, in the kernel shape the autotuner picks on
+    # the first batch of draws
     draws = draw(rng, a.train, 4000)
+    shape = {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.25}
+    if not a.no_tune:
+        first = [dict(n_beads=a.n_beads, volume_fraction=d["vf"], eps_het=d["eps_het"], eps_lam=d["eps_lam"],
+                      het_fraction=0.3, lad_fraction=0.1, seed=d["seed"]) for d in draws[:a.batch]]
+        common = {"timestep": 0.001, "contact_interval": a.interval, "contact_cutoff": 1.5, "contact_bins": BINS,
+                  "radial_bins": 32, "graph_steps": 100}
+        shape, _ = tune_batch(first, common, log=log)
     X = features(draws, a.n_beads)
     Y, R, t_sim = simulate(draws, a.n_beads, a.warmup, a.steps, a.interval, a.batch, shape, log, a.replicas)
     n_val = max(a.train // 8, 1)
