@@ -580,10 +580,11 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const int part = LANES > 1 ? lane % LANES : 0, slot = LANES > 1 ? lane / LANES : lane;
   const int t = (xcd_block() * (BEAD_BLOCK / 64) + wave) * BPW + slot;
-  const bool active = slot < BPW && t < p.N;
-  const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const bool in_range = slot < BPW && t < p.N;
+  const float4 me = in_range ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const uint32_t w = wbits(me);
   const uint32_t i = w & ID_MASK;
+  const bool active = in_range && i < (uint32_t)p.N; // a sentinel slot holds no bead
   const float rx = me.x, ry = me.y, rz = me.z;
   if constexpr (BATCH) load_system(p, q, active ? (uint32_t)t : 0u);
 
@@ -970,9 +971,9 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
       t = t0 + pos;
       me = s_me[pos];
     }
-    const bool active = t < p.N;
     const uint32_t w = wbits(me);
     const uint32_t i = w & ID_MASK;
+    const bool active = t < p.N && i < (uint32_t)p.N; // a sentinel slot holds no bead
     if constexpr (BATCH) load_system(p, q, active ? (uint32_t)t : 0u);
     // the stage input is needed only at the end: load it now
     float4 in = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -998,8 +999,8 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
     float kx = 0.0f, ky = 0.0f, kz = 0.0f;
     for (int rd = 0; rd < ROUNDS; ++rd) {
       const int t = wbase + rd * (64 / C_LANES) + gw;
-      const bool active = t < p.N;
-      const float4 me = active ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const float4 me = t < p.N ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const bool active = t < p.N && (wbits(me) & ID_MASK) < (uint32_t)p.N;
       if constexpr (BATCH) load_system(p, q, active ? (uint32_t)t : 0u);
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
       bal_pair_sum<MODE, BATCH, R>(p, q, s_list[grp], active, (uint32_t)t, sub, lane, me, fx, fy, fz);
@@ -1015,6 +1016,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
     const float4 me = q.fspos[t];
     const uint32_t w = wbits(me);
     const uint32_t i = w & ID_MASK;
+    if (i >= (uint32_t)p.N) return;
     if constexpr (BATCH) load_system(p, q, (uint32_t)t);
     if (q.fbond) {
       const float4 fb = q.fbond[i];
@@ -1168,6 +1170,14 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
       J.spos[start + rank] = v;
       if (J.soi) J.soi[start + rank] = e;
     }
+  }
+  // A cell with more members than its bucket holds (flagged above; the host
+  // then grows the buckets and rebuilds) keeps its full slot range: the slots
+  // past the stored members get a sentinel that every reader skips (index
+  // ID_MASK, far outside any grid) instead of stale memory.
+  for (uint32_t k = m; k < cnt; ++k) {
+    J.spos[start + k] = make_float4(1e18f, 1e18f, 1e18f, __uint_as_float(ID_MASK));
+    if (J.soi) J.soi[start + k] = ID_MASK;
   }
 }
 
