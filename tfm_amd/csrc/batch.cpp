@@ -58,6 +58,7 @@ struct BatchGrid {
 
 struct ccp_batch {
   uint32_t n_sys = 0, N = 0, bins = 128, rd_bins = 32, cinterval = 0, graph_steps = 64;
+  uint32_t spf = 0; // relaxation steps between overlap counts of the initial conditions
   bool integrate_pass = false;
   std::vector<std::string> text;     // parameter text of each system alone
   std::vector<SysParams> sys;
@@ -76,10 +77,11 @@ struct ccp_batch {
   // device
   int dev = 0;
   hipStream_t stream = nullptr;
-  DevBuf<float4> r, er, pstar, fpair, znoise, fbond;
+  DevBuf<float4> r, er, pstar, fpair, znoise, fbond, rsave;
   DevBuf<SysParams> dsys;
   DevBuf<uint32_t> dslot, flags, cmaps;
   DevBuf<uint64_t> dstep;
+  DevBuf<unsigned long long> dover; // overlap count per system
   DevBuf<double> obs;
   BatchGrid grid;
   hipGraphExec_t graphs[2] = {nullptr, nullptr};
@@ -94,12 +96,13 @@ struct ccp_batch {
   }
   ~ccp_batch() {
     drop_graphs();
-    DevBuf<float4> *f4[] = {&r, &er, &pstar, &fpair, &znoise, &fbond};
+    DevBuf<float4> *f4[] = {&r, &er, &pstar, &fpair, &znoise, &fbond, &rsave};
     for (auto *b : f4) b->free_();
     DevBuf<uint32_t> *u4[] = {&dslot, &flags, &cmaps};
     for (auto *b : u4) b->free_();
     dsys.free_();
     dstep.free_();
+    dover.free_();
     obs.free_();
     grid.free_();
     if (stream) (void)hipStreamDestroy(stream);
@@ -160,17 +163,26 @@ struct ccp_batch {
       hip_check(hipGetLastError());
       grid.clean ^= 1;
       if (!take_overflow()) break;
-      if (grid.g.cap >= 4096) throw Error(CCP_ERR_STATE, "cell_capacity exceeded; raise cell_capacity");
-      hip_check(hipStreamSynchronize(stream));
-      drop_graphs();
-      grid.g.cap *= 2;
-      kp.gf.cap = grid.g.cap;
-      for (auto &s : sys) s.g.cap = grid.g.cap;
-      hip_check(hipMemcpy(dsys.p, sys.data(), sys.size() * sizeof(SysParams), hipMemcpyHostToDevice));
-      grid.bucket.free_();
-      grid.bucket.alloc((size_t)grid.g.n_c * grid.g.cap);
+      grow();
     }
     grid.valid = true;
+  }
+  // double the cells' storage (the grid is rebuilt on its next use)
+  void grow() {
+    if (grid.g.cap >= 4096) throw Error(CCP_ERR_STATE, "cell_capacity exceeded; raise cell_capacity");
+    hip_check(hipStreamSynchronize(stream));
+    drop_graphs();
+    grid.g.cap *= 2;
+    kp.gf.cap = grid.g.cap;
+    for (auto &s : sys) s.g.cap = grid.g.cap;
+    upload_sys();
+    grid.bucket.free_();
+    grid.bucket.alloc((size_t)grid.g.n_c * grid.g.cap);
+    grid.valid = false;
+  }
+  void upload_sys() {
+    hip_check(hipMemcpyAsync(dsys.p, sys.data(), sys.size() * sizeof(SysParams), hipMemcpyHostToDevice, stream));
+    hip_check(hipStreamSynchronize(stream));
   }
   void sync_step() {
     if (dstep_dirty) {
@@ -184,19 +196,28 @@ struct ccp_batch {
     return NoiseJob{znoise.p, dstep.p, ahead, N, 0u, 0u, dsys.p, dslot.p, (int)n_sys};
   }
   bool bond_pass() const { return kp.lanes == 8 && !integrate_pass; }
-  BondJob bond_job(const float4 *pos) const {
-    return BondJob{fbond.p, pos, N, MODE_LJ, dsys.p, dslot.p, (int)n_sys};
+  BondJob bond_job(const float4 *pos, int mode) const {
+    return BondJob{fbond.p, pos, N, mode, dsys.p, dslot.p, (int)n_sys};
+  }
+  // the WCA/LJ set, or the noise-free soft-repulsion relaxation of the
+  // initial conditions (as ccp_sim's mode_params)
+  KParams mode_params(int mode) const {
+    KParams k = kp;
+    if (mode == MODE_ICG) k.sd = 0.0f;
+    return k;
   }
   // before the first stage of a run: the current step's normals (unless held
   // already) and the bonded terms of the current positions
-  void prime_terms() {
+  void prime_terms(int mode = MODE_LJ) {
     sync_step();
     const NoiseJob nj = noise_job(0);
-    const BondJob bj = bond_job(r.p);
-    launch_bead_terms(stream, noise_step != step ? &nj : nullptr, kp, bond_pass() ? &bj : nullptr);
+    const BondJob bj = bond_job(r.p, mode);
+    const KParams k = mode_params(mode);
+    launch_bead_terms(stream, noise_step != step ? &nj : nullptr, k, bond_pass() ? &bj : nullptr);
     noise_step = step;
   }
-  void launch_stage(int stage) {
+  void launch_stage(int stage, int mode = MODE_LJ) {
+    const KParams k = mode_params(mode);
     KPtrs q{};
     q.fspos = grid.spos.p;
     q.fcstart = grid.cstart.p;
@@ -214,14 +235,14 @@ struct ccp_batch {
     q.sys = dsys.p;
     q.sys_slot = dslot.p;
     q.n_sys = (int)n_sys;
-    launch_force(stream, MODE_LJ, stage, kp, q);
+    launch_force(stream, mode, stage, k, q);
     OrderJob j{grid.g, grid.count.p, grid.bs(grid.clean), grid.bs(grid.clean ^ 1), grid.bucket.p,
                grid.spos.p, nullptr, grid.cstart.p};
     // with the order pass: the next step's normals after the first stage,
     // the bonded terms of q.out0 after both
     const NoiseJob nj = noise_job(1);
-    const BondJob bj = bond_job(q.out0);
-    launch_order(stream, j, nullptr, flags.p, stage == 2 ? dstep.p : nullptr, stage == 1 ? &nj : nullptr, &kp,
+    const BondJob bj = bond_job(q.out0, mode);
+    launch_order(stream, j, nullptr, flags.p, stage == 2 ? dstep.p : nullptr, stage == 1 ? &nj : nullptr, &k,
                  bond_pass() ? &bj : nullptr);
     grid.clean ^= 1;
   }
@@ -277,6 +298,76 @@ struct ccp_batch {
       for (uint32_t k = 0; k < rd_bins; ++k) rd_sum[(size_t)s * rd_bins + k] += a[per * s + 6 + 2 * rd_bins + k] / shell;
     }
     ++n_samples;
+  }
+  // Overlapping non-bonded pairs (closer than the initial-overlap distance)
+  // of each listed system, on the batch grid.
+  std::vector<unsigned long long> count_overlaps(const std::vector<uint32_t> &which) {
+    ensure_grid();
+    hip_check(hipMemsetAsync(dover.p, 0, n_sys * sizeof(unsigned long long), stream));
+    for (uint32_t s : which) {
+      const SysParams &S = sys[s];
+      KParams k = kp;
+      k.N = (int)S.n;
+      k.gf = S.g;
+      launch_overlaps(stream, k, grid.spos.p, grid.cstart.p, S.slot, MIS * MIS, dover.p + s);
+    }
+    hip_check(hipGetLastError());
+    std::vector<unsigned long long> h(n_sys);
+    hip_check(hipMemcpyAsync(h.data(), dover.p, n_sys * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    hip_check(hipStreamSynchronize(stream));
+    return h;
+  }
+  void set_frozen(const std::vector<uint32_t> &active) {
+    for (auto &S : sys) S.frozen = 1;
+    for (uint32_t s : active) sys[s].frozen = 0;
+    upload_sys();
+  }
+  // chrsim::generate_initial_condition's relaxation (chrsim.cu:470-522) for
+  // every system at once: rounds of spf noise-free soft-repulsion steps until
+  // a system has no overlapping pair. A finished system is frozen (zero force:
+  // its beads keep their positions) while the others go on. A round in which
+  // a cell overflowed is undone and redone with larger cells, as ccp_sim's
+  // relaxation does, so each system ends exactly as it would alone.
+  void relax() {
+    std::vector<uint32_t> active;
+    for (uint32_t s = 0; s < n_sys; ++s) active.push_back(s);
+    auto still = [&](const std::vector<unsigned long long> &po) {
+      std::vector<uint32_t> next;
+      for (uint32_t s : active)
+        if (po[s] > 0) next.push_back(s);
+      return next;
+    };
+    active = still(count_overlaps(active));
+    set_frozen(active);
+    uint32_t rounds = 0;
+    while (!active.empty()) {
+      if (++rounds > 10000) throw Error(CCP_ERR_STATE, "unable to remove bead overlaps");
+      hip_check(hipMemcpyAsync(rsave.p, r.p, (size_t)N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+      for (;;) {
+        ensure_grid();
+        prime_terms(MODE_ICG);
+        for (uint32_t k = 0; k < spf; ++k) {
+          launch_stage(1, MODE_ICG);
+          launch_stage(2, MODE_ICG);
+        }
+        hip_check(hipGetLastError());
+        step += spf;
+        noise_step = step;
+        if (!take_overflow()) break;
+        hip_check(hipMemcpyAsync(r.p, rsave.p, (size_t)N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        step -= spf;
+        dstep_dirty = true;
+        grow();
+      }
+      check_flags();
+      const std::vector<uint32_t> next = still(count_overlaps(active));
+      if (next.size() != active.size()) {
+        active = next;
+        set_frozen(active);
+      }
+    }
+    for (auto &S : sys) S.frozen = 0;
+    upload_sys();
   }
   void run_steps(uint64_t n) {
     ensure_grid();
@@ -377,6 +468,7 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
         b->contact_rc = one.contact_rc;
         b->cinterval = one.cinterval;
         b->graph_steps = one.graph_steps;
+        b->spf = one.spf;
         b->integrate_pass = one.integrate_pass;
       } else if (one.kp.lanes != b->kp.lanes || one.kp.reach != b->kp.reach || one.gf.g.csl != b->kp.gf.csl ||
                  one.gf.g.cap != b->kp.gf.cap || one.kp.dt != b->kp.dt || one.kp.sd != b->kp.sd) {
@@ -410,6 +502,7 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
     b->r.alloc(b->N);
     b->er.alloc(b->N);
     b->pstar.alloc(b->N);
+    b->rsave.alloc(b->N);
     b->znoise.alloc(b->N);
     if (b->bond_pass()) b->fbond.alloc(b->N);
     if (b->kp.lanes == 8 && b->integrate_pass) b->fpair.alloc(b->N);
@@ -418,6 +511,7 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
     b->flags.alloc(FLAG_COUNT);
     b->cmaps.alloc((size_t)n_sys * b->bins * b->bins);
     b->dstep.alloc(1);
+    b->dover.alloc(n_sys);
     b->obs.alloc((6 + 3 * (size_t)b->rd_bins) * n_sys);
     b->grid.alloc(b->N);
     hip_check(hipMemcpy(b->dsys.p, b->sys.data(), n_sys * sizeof(SysParams), hipMemcpyHostToDevice));
@@ -458,19 +552,35 @@ int ccp_batch_system_params(const ccp_batch *b, uint32_t s, char *buf, size_t le
   return CCP_OK;
 }
 
+// Each system's initial condition is the one a ccp_sim of its parameters
+// generates alone: its host walk, then the relaxation, run here for all
+// systems together on the batch grid.
 int ccp_batch_generate_initial_conditions(ccp_batch *b) {
   return bguarded(b, [&] {
     for (uint32_t s = 0; s < b->n_sys; ++s) {
-      std::unique_ptr<ccp_sim, void (*)(ccp_sim *)> one(ccp_create_logged(b->text[s].c_str(), quiet, nullptr),
-                                                         ccp_destroy);
-      if (!one) throw Error(CCP_ERR_PARAM, ccp_last_error(nullptr));
-      if (ccp_generate_initial_condition(one.get()) != CCP_OK) throw Error(CCP_ERR_STATE, one->err);
+      ccp_sim one;
+      one.log_fn = quiet;
+      ParMap par(b->text[s].c_str());
+      one.configure(par);
+      HostRng g(one.seed);
+      one.set_lbs_positions(g);
+      one.set_particle_types(g);
+      if (one.replica) {
+        HostRng w(one.seed, one.replica);
+        one.set_particle_positions(w);
+      } else {
+        one.set_particle_positions(g);
+      }
       const uint32_t o = b->slot[s];
-      std::copy(one->hr.begin(), one->hr.end(), b->hr.begin() + 3 * (size_t)o);
-      std::copy(one->hfl.begin(), one->hfl.end(), b->hfl.begin() + o);
+      std::copy(one.hr.begin(), one.hr.end(), b->hr.begin() + 3 * (size_t)o);
+      std::copy(one.hfl.begin(), one.hfl.end(), b->hfl.begin() + o);
     }
     hip_check(hipSetDevice(b->dev));
     b->upload();
+    b->step = 0;
+    b->dstep_dirty = true;
+    b->relax();
+    b->download();
     b->step = 0;
     b->dstep_dirty = true;
   });

@@ -693,7 +693,7 @@ void ccp_sim::set_particle_positions(HostRng &g) {
 uint64_t ccp_sim::count_overlaps(float dmin) {
   ensure_grids();
   hip_check(hipMemsetAsync(dcount.p, 0, sizeof(unsigned long long), stream));
-  launch_overlaps(stream, kp, gf.spos.p, gf.cstart.p, dmin * dmin, dcount.p);
+  launch_overlaps(stream, kp, gf.spos.p, gf.cstart.p, 0u, dmin * dmin, dcount.p);
   unsigned long long c = 0;
   hip_check(hipMemcpyAsync(&c, dcount.p, sizeof(c), hipMemcpyDeviceToHost, stream));
   hip_check(hipStreamSynchronize(stream));
@@ -719,8 +719,16 @@ void ccp_sim::generate_initial_condition() {
   while (po > 0) {
     progress((float)(100.0 * (ipo - po) / ipo));
     if (++rounds > 10000) throw Error(CCP_ERR_STATE, "unable to remove bead overlaps");
-    run_steps(spf, MODE_ICG);
-    if (take_overflow()) { // clumps of the raw walk: widen the cells' storage and go on
+    // a round in which a cell overflowed (clumps of the raw walk) is undone
+    // and redone with wider cells: the result does not depend on the cells'
+    // storage (the batch engine relaxes many systems with shared cells)
+    hip_check(hipMemcpyAsync(aux.p, r.p, (size_t)N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    for (;;) {
+      run_steps(spf, MODE_ICG);
+      if (!take_overflow()) break;
+      hip_check(hipMemcpyAsync(r.p, aux.p, (size_t)N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+      step -= spf;
+      dstep_dirty = true;
       grow_capacity();
     }
     check_flags();

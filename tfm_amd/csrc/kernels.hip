@@ -530,6 +530,7 @@ __device__ __forceinline__ void load_system(KParams &p, const KPtrs &q, uint32_t
   p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
   p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
   p.gf = S.g;
+  p.frozen = S.frozen;
 }
 
 // Unit normal triple of bead g (index over all systems) for the job's step.
@@ -635,6 +636,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force(KParams p, KPtrs q) {
   } else {
     if (!active) return;
   }
+  if (p.frozen) fx = fy = fz = 0.0f; // with no noise (relaxation) the bead stays put
   if (STAGE != 0 && p.fcap2 > 0.0f) {
     // optional safeguard of the WCA/LJ set: |F| <= fcap
     const float m2 = fx * fx + fy * fy + fz * fz;
@@ -680,6 +682,7 @@ template <int STAGE> __device__ __forceinline__ float4 stage_input(const KParams
 template <int STAGE>
 __device__ __forceinline__ void stage_update(const KParams &p, const KPtrs &q, uint32_t i, uint32_t w, float4 me,
                                              float fx, float fy, float fz, float4 in) {
+  if (p.frozen) fx = fy = fz = 0.0f; // with no noise (relaxation) the bead stays put
   if (STAGE != 0 && p.fcap2 > 0.0f) {
     const float m2 = fx * fx + fy * fy + fz * fz;
     if (m2 > p.fcap2) {
@@ -1232,11 +1235,11 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4
 // Non-bonded pairs (|i - j| > 1) closer than dmin <= fine cell side
 // (particle_overlaps, chrsim.cu:805-820), each pair counted once.
 __global__ __launch_bounds__(BEAD_BLOCK) void k_overlaps(KParams p, const float4 *spos, const uint32_t *cstart,
-                                                         float dmin2, unsigned long long *count) {
+                                                         uint32_t slot0, float dmin2, unsigned long long *count) {
   const int t = blockIdx.x * BEAD_BLOCK + threadIdx.x;
   uint32_t n = 0;
   if (t < p.N) {
-    const float4 me = spos[t];
+    const float4 me = spos[slot0 + t];
     const uint32_t i = wbits(me) & ID_MASK;
     Rows R;
     neighbour_rows(p.gf, cstart, me.x, me.y, me.z, R);
@@ -1503,8 +1506,10 @@ template <int MODE, bool BATCH> static void force_shape(hipStream_t s, int stage
 void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q) {
   if (p.N <= 0) return;
   if (q.sys) {
-    // batches: the WCA/LJ set on one all-bead grid per system
-    force_shape<MODE_LJ, true>(s, stage, p, q);
+    // batches: the WCA/LJ set (or the relaxation of the initial conditions)
+    // on one all-bead grid per system
+    if (mode == MODE_ICG) force_shape<MODE_ICG, true>(s, stage, p, q);
+    else force_shape<MODE_LJ, true>(s, stage, p, q);
     return;
   }
   if (p.split && mode != MODE_ICG) {
@@ -1525,10 +1530,10 @@ void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const 
   k_contacts<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, map, rad, slot0, id0);
 }
 
-void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, float dmin2,
-                     unsigned long long *count) {
+void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t slot0,
+                     float dmin2, unsigned long long *count) {
   if (p.N <= 0) return;
-  k_overlaps<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, dmin2, count);
+  k_overlaps<<<blocks_for(p.N, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(p, spos, cstart, slot0, dmin2, count);
 }
 
 void launch_noise(hipStream_t s, const KParams &p, uint64_t step, float4 *out) {

@@ -49,6 +49,7 @@ struct SysParams {
   uint32_t seed_lo, seed_hi;
   uint32_t slot, n;
   GridGeom g;
+  int frozen; // initial-condition relaxation done: zero force, the beads keep their positions
 };
 
 // Device buffers of one grid. count/blocksum/bucket receive the binning of
@@ -71,6 +72,7 @@ struct KParams {
   int lanes;   // lanes per bead of the force kernel: 2 reach + 1 (one per z-plane), 1, or 8 (balanced list)
   int group;   // 8 lanes: within a workgroup, beads with the long (het) cutoff take the first wavefronts
   int rounds;  // 8 lanes: beads per lane group of a wavefront (1, 2, 4 or 8), eight per round of the pair phase
+  int frozen;  // batches: the bead's system is frozen (set per bead from its SysParams)
   // nucleus geometry (cngeom, chrdat.cuh:48-69)
   float R_n, R_o, R_b, noc, boc, nod, bod;
   // bonds and bending
@@ -174,8 +176,9 @@ void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KP
 // id0) on p.gf, binned by (index - id0) / contact_bin_size.
 void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map,
                      uint32_t slot0 = 0, uint32_t id0 = 0);
-void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, float dmin2,
-                     unsigned long long *count);
+// slots [slot0, slot0 + p.N) of spos, pairs counted on p.gf
+void launch_overlaps(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t slot0,
+                     float dmin2, unsigned long long *count);
 void launch_noise(hipStream_t s, const KParams &p, uint64_t step, float4 *out);
 void launch_observables(hipStream_t s, int N, double R_n, double nod, double a, int n_b, const float4 *pos,
                         double *out);
