@@ -78,13 +78,16 @@ def tune(params, positions, flags, cutoff, steps=200, warmup=50, cache=None, log
     return best, table
 
 
-def tune_batch(systems, common, cutoff=2.5, steps=100, warmup=20, log=None):
+def tune_batch(systems, common, cutoff=2.5, steps=100, warmup=20, settle=2000, log=None):
     """Time every candidate shape on a batch of systems (tfm_amd.Batch) from
-    the same initial conditions; returns (best overrides, [(overrides, ms per
-    step), ...])."""
+    the same state: the initial conditions after `settle` steps, closer to
+    the long run than the fresh relaxed walks. Returns (best overrides,
+    [(overrides, ms per step), ...])."""
     ref = Batch(systems, common)
     try:
         ref.generate_initial_conditions()
+        if settle:
+            ref.run_steps(settle)
         xyz, fl = ref.state()
     finally:
         ref.close()
