@@ -274,28 +274,19 @@ struct ccp_batch {
     }
     hip_check(hipGetLastError());
   }
+  // every system's contacts and radial density, one launch each
   void sample() {
-    const size_t per = 6 + 3 * (size_t)rd_bins;
-    hip_check(hipMemsetAsync(obs.p, 0, per * n_sys * sizeof(double), stream));
-    for (uint32_t s = 0; s < n_sys; ++s) {
-      const SysParams &S = sys[s];
-      KParams k = kp;
-      k.N = (int)S.n;
-      k.gf = S.g;
-      k.contact_rc = contact_rc;
-      k.contact_bins = (int)bins;
-      k.contact_bin_size = (int)((S.n + bins - 1) / bins);
-      launch_contacts(stream, k, grid.spos.p, grid.cstart.p, cmaps.p + (size_t)s * bins * bins, S.slot, S.slot);
-      launch_observables(stream, (int)S.n, (double)S.R_n, (double)S.nod, (double)RCO / 2.0, (int)rd_bins, r.p + S.slot,
-                         obs.p + per * s);
-    }
+    hip_check(hipMemsetAsync(obs.p, 0, (size_t)n_sys * rd_bins * sizeof(double), stream));
+    launch_contacts_batch(stream, dsys.p, dslot.p, (int)n_sys, (int)N, contact_rc, bins, grid.spos.p, grid.cstart.p,
+                          cmaps.p);
+    launch_radial_batch(stream, dsys.p, dslot.p, (int)n_sys, (int)N, (double)RCO / 2.0, (int)rd_bins, r.p, obs.p);
     hip_check(hipGetLastError());
-    std::vector<double> a(per * n_sys);
+    std::vector<double> a((size_t)n_sys * rd_bins);
     hip_check(hipMemcpyAsync(a.data(), obs.p, a.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     hip_check(hipStreamSynchronize(stream));
     for (uint32_t s = 0; s < n_sys; ++s) {
       const double R = sys[s].R_n, shell = 4.0 / 3.0 * M_PI * R * R * R / rd_bins;
-      for (uint32_t k = 0; k < rd_bins; ++k) rd_sum[(size_t)s * rd_bins + k] += a[per * s + 6 + 2 * rd_bins + k] / shell;
+      for (uint32_t k = 0; k < rd_bins; ++k) rd_sum[(size_t)s * rd_bins + k] += a[(size_t)s * rd_bins + k] / shell;
     }
     ++n_samples;
   }

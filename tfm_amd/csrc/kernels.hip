@@ -1193,18 +1193,15 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_bead_terms(NoiseJob nj, KParams 
 
 // Contact counts on the fine grid: pairs i < j with d < rc within
 // rad = ceil(rc / csl) cells, FP32 without fused operations so the decision
-// matches the oracle bit for bit.
-__global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4 *spos, const uint32_t *cstart,
-                                                         uint32_t *map, int rad, uint32_t slot0, uint32_t id0) {
+// matches the oracle bit for bit. Bead i (index within its system, whose
+// first bead is id0) counts its partners j > i into bins of bin_size beads.
+__device__ __forceinline__ void bead_contacts(const GridGeom &g, const float4 *spos, const uint32_t *cstart,
+                                              float4 me, uint32_t i, uint32_t id0, int rad, float rc, uint32_t nb,
+                                              uint32_t bin_size, uint32_t *map) {
 #pragma clang fp contract(off)
-  const int t = xcd_block() * BEAD_BLOCK + threadIdx.x;
-  if (t >= p.N) return;
-  const GridGeom &g = p.gf;
-  const float4 me = spos[slot0 + t];
-  const uint32_t i = (wbits(me) & ID_MASK) - id0;
   bool outside = false;
   const int cx = cell_coord(me.x, g, outside), cy = cell_coord(me.y, g, outside), cz = cell_coord(me.z, g, outside);
-  const uint32_t nb = (uint32_t)p.contact_bins, bi = i / (uint32_t)p.contact_bin_size;
+  const uint32_t bi = i / bin_size;
   for (int dz = -rad; dz <= rad; ++dz) {
     const int z = cz + dz;
     if (z < 0 || z >= g.cps) continue;
@@ -1220,8 +1217,8 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4
         const float dx = me.x - o.x, dy2 = me.y - o.y, dz2 = me.z - o.z;
         const float xx = dx * dx, yy = dy2 * dy2, zz = dz2 * dz2;
         const float d2 = (xx + yy) + zz;
-        if (sqrtf(d2) < p.contact_rc) {
-          const uint32_t bj = j / (uint32_t)p.contact_bin_size;
+        if (sqrtf(d2) < rc) {
+          const uint32_t bj = j / bin_size;
           if (bi < nb && bj < nb) {
             atomicAdd(map + (size_t)bi * nb + bj, 1u);
             atomicAdd(map + (size_t)bj * nb + bi, 1u);
@@ -1230,6 +1227,32 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4
       }
     }
   }
+}
+
+__global__ __launch_bounds__(BEAD_BLOCK) void k_contacts(KParams p, const float4 *spos, const uint32_t *cstart,
+                                                         uint32_t *map, int rad, uint32_t slot0, uint32_t id0) {
+  const int t = xcd_block() * BEAD_BLOCK + threadIdx.x;
+  if (t >= p.N) return;
+  const float4 me = spos[slot0 + t];
+  bead_contacts(p.gf, spos, cstart, me, (wbits(me) & ID_MASK) - id0, id0, rad, p.contact_rc,
+                (uint32_t)p.contact_bins, (uint32_t)p.contact_bin_size, map);
+}
+
+// The contacts of every system of a batch in one launch: slot t belongs to
+// the system whose slot range holds it; that system's map is maps + s bins^2
+// and its bins hold ceil(n / bins) beads.
+__global__ __launch_bounds__(BEAD_BLOCK) void k_contacts_batch(const SysParams *sys, const uint32_t *sys_slot,
+                                                               int n_sys, int n, float rc, uint32_t bins,
+                                                               const float4 *spos, const uint32_t *cstart,
+                                                               uint32_t *maps) {
+  const int t = xcd_block() * BEAD_BLOCK + threadIdx.x;
+  if (t >= n) return;
+  const int s = system_of(sys_slot, n_sys, (uint32_t)t);
+  const SysParams &S = sys[s];
+  const float4 me = spos[t];
+  const int rad = (int)ceilf(rc / S.g.csl);
+  bead_contacts(S.g, spos, cstart, me, (wbits(me) & ID_MASK) - S.slot, S.slot, rad, rc, bins,
+                (S.n + bins - 1) / bins, maps + (size_t)s * bins * bins);
 }
 
 // Non-bonded pairs (|i - j| > 1) closer than dmin <= fine cell side
@@ -1268,6 +1291,31 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_noise(KParams p, uint64_t step, 
 // with z < nod, then the radial density of A, B and all beads (3 x n_b,
 // lens volume of a bead of radius a inside the sphere of each bin edge,
 // differenced bin by bin).
+// Volume of a bead of radius a at distance d from the centre inside each of
+// n_b shells of equal volume (edges R_n cbrt((k + 1) / n_b)), as lens
+// volumes at the outer edge differenced bin by bin; add(k, net) receives the
+// nonzero differences.
+template <class F> __device__ __forceinline__ void lens_bins(double d, double a, double R_n, int n_b, F &&add) {
+  const double vb = 4.0 / 3.0 * M_PI * a * a * a;
+  double prev = 0.0;
+  for (int k = 0; k < n_b; ++k) {
+    const double rb = R_n * cbrt((k + 1.0) / n_b);
+    double v;
+    if (d - a > rb) v = 0.0;
+    else if (d + a < rb) v = vb;
+    else {
+      const double ca = (d * d + a * a - rb * rb) / (2.0 * d * a);
+      const double cb = (rb * rb + d * d - a * a) / (2.0 * rb * d);
+      v = M_PI / 3.0 * a * a * a * (2.0 + ca) * (1.0 - ca) * (1.0 - ca) +
+          M_PI / 3.0 * rb * rb * rb * (2.0 + cb) * (1.0 - cb) * (1.0 - cb);
+    }
+    const double net = v - prev;
+    prev = v;
+    if (net != 0.0) add(k, net);
+    if (v == vb) break; // the rest of the bins see no change
+  }
+}
+
 __global__ __launch_bounds__(BEAD_BLOCK) void k_observables(int N, double R_n, double nod, double a, int n_b,
                                                             const float4 *pos, double *out) {
   extern __shared__ double s_acc[];
@@ -1290,32 +1338,29 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_observables(int N, double R_n, d
       atomicAdd(&s_acc[4], 0.5 * (3.0 * c * c - 1.0));
     }
     if (z < nod) atomicAdd(&s_acc[5], 1.0);
-    const double d = sqrt(d2), vb = 4.0 / 3.0 * M_PI * a * a * a;
     const int t = (wbits(q) & W_HET) ? 0 : 1;
-    double prev = 0.0;
-    for (int k = 0; k < n_b; ++k) {
-      const double rb = R_n * cbrt((k + 1.0) / n_b);
-      double v;
-      if (d - a > rb) v = 0.0;
-      else if (d + a < rb) v = vb;
-      else {
-        const double ca = (d * d + a * a - rb * rb) / (2.0 * d * a);
-        const double cb = (rb * rb + d * d - a * a) / (2.0 * rb * d);
-        v = M_PI / 3.0 * a * a * a * (2.0 + ca) * (1.0 - ca) * (1.0 - ca) +
-            M_PI / 3.0 * rb * rb * rb * (2.0 + cb) * (1.0 - cb) * (1.0 - cb);
-      }
-      const double net = v - prev;
-      prev = v;
-      if (net != 0.0) {
-        atomicAdd(&s_acc[6 + t * n_b + k], net);
-        atomicAdd(&s_acc[6 + 2 * n_b + k], net);
-      }
-      if (v == vb) break; // the rest of the bins see no change
-    }
+    lens_bins(sqrt(d2), a, R_n, n_b, [&](int k, double net) {
+      atomicAdd(&s_acc[6 + t * n_b + k], net);
+      atomicAdd(&s_acc[6 + 2 * n_b + k], net);
+    });
   }
   __syncthreads();
   for (int k = tid; k < n_acc; k += BEAD_BLOCK)
     if (s_acc[k] != 0.0) atomicAdd(&out[k], s_acc[k]);
+}
+
+// Radial density of every system of a batch (the all-bead row of
+// k_observables), one lane per bead, FP64 sums into out[s n_b + k].
+__global__ __launch_bounds__(BEAD_BLOCK) void k_radial_batch(const SysParams *sys, const uint32_t *sys_slot,
+                                                             int n_sys, int n, double a, int n_b, const float4 *pos,
+                                                             double *out) {
+  const int i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const int s = system_of(sys_slot, n_sys, (uint32_t)i);
+  const float4 q = pos[i];
+  const double x = q.x, y = q.y, z = q.z;
+  double *o = out + (size_t)s * n_b;
+  lens_bins(sqrt(x * x + y * y + z * z), a, (double)sys[s].R_n, n_b, [&](int k, double net) { atomicAdd(o + k, net); });
 }
 
 // ccp's d(s) and 1000 P(s) of one chromosome (calc_sd_cp, chrana.cu:13-45):
@@ -1521,6 +1566,19 @@ void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KP
   if (mode == MODE_ICG) force_shape<MODE_ICG, false>(s, stage, p, q);
   else if (mode == MODE_CCP) force_shape<MODE_CCP, false>(s, stage, p, q);
   else force_shape<MODE_LJ, false>(s, stage, p, q);
+}
+
+void launch_contacts_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n, float rc,
+                           uint32_t bins, const float4 *spos, const uint32_t *cstart, uint32_t *maps) {
+  if (n <= 0) return;
+  k_contacts_batch<<<blocks_for(n, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(sys, sys_slot, n_sys, n, rc, bins, spos, cstart,
+                                                                    maps);
+}
+
+void launch_radial_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n, double a,
+                         int n_b, const float4 *pos, double *out) {
+  if (n <= 0) return;
+  k_radial_batch<<<blocks_for(n, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(sys, sys_slot, n_sys, n, a, n_b, pos, out);
 }
 
 void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map,

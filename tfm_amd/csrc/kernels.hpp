@@ -187,6 +187,12 @@ void launch_sd_cp(hipStream_t s, const float4 *pos, uint32_t N, uint32_t lsdcp, 
 void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm, double *sum);
 void launch_bin_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n,
                       const float4 *pos, uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags);
+// every system of a batch in one launch: contact maps (maps + s bins^2) and
+// the all-bead radial density (out + s n_b, FP64 sums)
+void launch_contacts_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n, float rc,
+                           uint32_t bins, const float4 *spos, const uint32_t *cstart, uint32_t *maps);
+void launch_radial_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n, double a,
+                         int n_b, const float4 *pos, double *out);
 void launch_close_sums(hipStream_t s, double *sum, double *m1, double *m2, uint32_t n, double n_dp);
 
 } // namespace ccpmi
