@@ -121,6 +121,14 @@ int ccp_read_lbs_txt(ccp_sim *h, const char *path);
  * chrdat::write_frame_bin :168-201). trr_path NULL runs without output. */
 int ccp_run_simulation(ccp_sim *h, const char *trr_path);
 
+/* Frames of spf steps handed to the caller instead of a file: after each
+ * frame the host positions (xyz, n beads) go to cb together with the frame
+ * index and time, which advance as in ccp_run_simulation. A nonzero return
+ * from cb stops the run (status CCP_OK). Replaces chrsim::run_simulation's
+ * frame writes (chrsim.cu:558-593) for callers that consume frames in memory. */
+typedef int (*ccp_frame_fn)(uint32_t i_f, float t, const float *xyz, size_t n, void *user);
+int ccp_run_frames(ccp_sim *h, uint32_t n_frames, ccp_frame_fn cb, void *user);
+
 /* The step loop alone (chrsim.cu:570-580): n stochastic-Heun steps with
  * contact sampling every contact_interval steps; no host transfer. */
 int ccp_run_steps(ccp_sim *h, uint64_t n);

@@ -267,3 +267,31 @@ def test_type_grouping_is_bit_exact():
     assert np.array_equal(a.positions, b.positions)
     a.close()
     b.close()
+
+
+def test_run_frames_callback():
+    # frames go to the callback in memory: indices and times advance as in
+    # run_simulation, the positions are the frame's, and a true return stops
+    step("run_frames")
+    a = small_sim()
+    a.generate_initial_condition()
+    a.set_time(0, 0.0, 0)
+    seen = []
+
+    def take(i_f, t, xyz):
+        seen.append((i_f, t, xyz))
+        return len(seen) == 2
+
+    a.run_frames(5, take)
+    assert [s[0] for s in seen] == [1, 2]
+    assert abs(seen[1][1] - 2 * 50 * a.info.dt) < 1e-6
+    assert np.array_equal(seen[1][2], a.positions)
+    assert a.time[2] == 100
+    b = small_sim()
+    b.generate_initial_condition()
+    b.set_time(0, 0.0, 0)
+    b.run_steps(50)
+    b.synchronize()
+    assert np.array_equal(b.positions, seen[0][2])
+    a.close()
+    b.close()

@@ -22,7 +22,8 @@ FF_CCP, FF_ICG, FF_LJ = 0, 1, 2
 EXPORTS = (
     "ccp_create", "ccp_create_logged", "ccp_destroy", "ccp_last_error", "ccp_set_logger", "ccp_get_info",
     "ccp_generate_initial_condition", "ccp_save_checkpoint", "ccp_load_checkpoint", "ccp_write_frame_txt",
-    "ccp_read_frame_txt", "ccp_write_lbs_txt", "ccp_read_lbs_txt", "ccp_run_simulation", "ccp_run_steps",
+    "ccp_read_frame_txt", "ccp_write_lbs_txt", "ccp_read_lbs_txt", "ccp_run_simulation", "ccp_run_frames",
+    "ccp_run_steps",
     "ccp_synchronize", "ccp_get_time", "ccp_set_time", "ccp_get_positions", "ccp_set_positions",
     "ccp_get_types", "ccp_set_types", "ccp_get_flags", "ccp_set_flags", "ccp_get_lbs", "ccp_set_lbs",
     "ccp_get_chains", "ccp_compute_forces", "ccp_cell_list", "ccp_noise", "ccp_count_overlaps",
@@ -53,6 +54,8 @@ class Info(ctypes.Structure):
 
 
 LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
+FRAME_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_uint32, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                            ctypes.c_size_t, ctypes.c_void_p)
 
 
 class BatchSystem(ctypes.Structure):
@@ -92,6 +95,7 @@ def lib():
                   "ccp_write_lbs_txt", "ccp_read_lbs_txt", "ccp_run_simulation"):
             getattr(L, n).argtypes = [vp, ctypes.c_char_p]
         L.ccp_run_steps.argtypes = [vp, ctypes.c_uint64]
+        L.ccp_run_frames.argtypes = [vp, ctypes.c_uint32, FRAME_FN, vp]
         L.ccp_get_time.argtypes = [vp, u32p, f32p, P(ctypes.c_uint64)]
         L.ccp_set_time.argtypes = [vp, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64]
         L.ccp_get_positions.argtypes = [vp, f32p, sz]
@@ -213,6 +217,14 @@ class Simulation:
 
     def run_simulation(self, path=None):
         self._check(lib().ccp_run_simulation(self.h, None if path is None else str(path).encode()))
+
+    def run_frames(self, n_frames, fn):
+        """Frames of steps_per_frame steps; fn(i_f, t, xyz[N, 3]) after each
+        (a copy of the host positions); a true return value stops the run."""
+        def call(i_f, t, xyz, n, user):
+            return 1 if fn(i_f, t, np.ctypeslib.as_array(xyz, shape=(n, 3)).copy()) else 0
+        cb = FRAME_FN(call)
+        self._check(lib().ccp_run_frames(self.h, int(n_frames), cb, None))
 
     def write_frame_txt(self, path):
         self._check(lib().ccp_write_frame_txt(self.h, str(path).encode()))

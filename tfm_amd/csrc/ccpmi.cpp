@@ -1036,6 +1036,21 @@ int ccp_run_simulation(ccp_sim *h, const char *trr_path) {
   });
 }
 
+int ccp_run_frames(ccp_sim *h, uint32_t n_frames, ccp_frame_fn cb, void *user) {
+  return guarded(h, [&] {
+    need(cb != nullptr, "frame callback missing");
+    h->lgrid_valid = false;
+    for (uint32_t k = 0; k < n_frames; ++k) {
+      h->run_steps(h->spf, h->ff == CCP_FF_LJ ? MODE_LJ : MODE_CCP);
+      h->download_positions();
+      h->check_flags();
+      ++h->i_f;
+      h->t += h->spf * h->dt;
+      if (cb(h->i_f, h->t, h->hr.data(), h->N, user)) break;
+    }
+  });
+}
+
 int ccp_run_steps(ccp_sim *h, uint64_t n) {
   return guarded(h, [&] { h->run_steps(n, h->ff == CCP_FF_LJ ? MODE_LJ : MODE_CCP); });
 }
