@@ -36,6 +36,11 @@ namespace {
 __device__ __forceinline__ uint32_t wbits(float4 v) { return __float_as_uint(v.w); }
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// the element at byte offset `off` of `base` (a 32-bit offset: the load takes
+// the scalar base and the offset register as they are)
+template <class T> __device__ __forceinline__ T at_byte(const T *base, uint32_t off) {
+  return *(const T *)((const char *)base + off);
+}
 
 // ---------------------------------------------------------------- Philox4x32-10
 // Salmon, Moraes, Dror, Shaw (SC'11): counter-based, one call per bead and
@@ -415,13 +420,69 @@ __device__ __forceinline__ void pair_full(const KParams &p, float rx, float ry, 
     fac = (nb && d <= p.rco) ? fac : 0.0f;
   } else {
     // sigma = 1: 24 eps (2 r^-12 - r^-6) / r^2, WCA within 2^(1/6) for
-    // every pair, the LJ tail beyond it for het-het pairs
+    // every pair, the LJ tail beyond it for het-het pairs. The conditions
+    // are combined with & and | (no short circuit), so they stay lane masks
+    // instead of 0/1 values.
     const float u = fast_rcp(d2);
     const float s6 = u * u * u;
     const float base = s6 * fmaf(48.0f, s6, -24.0f) * u;
     const bool wca = d2 < p.rm2;
-    const bool on = nb && (wca || (hhi && d2 < p.rc_lj2));
+    const bool on = nb & (wca | (hhi & (d2 < p.rc_lj2)));
     fac = on ? (wca ? p.eps_wca : p.eps_het) * base : 0.0f;
+  }
+  fx += fac * dx;
+  fy += fac * dy;
+  fz += fac * dz;
+}
+
+// Per-bead operands of pair_full held in registers across a bead's candidate
+// loop. i - 1 is hidden from the compiler, which would otherwise rewrite
+// j - (i - 1) as (j - i) + 1, two operations per pair; both LJ well depths sit
+// in vector registers, since a select takes at most one scalar operand.
+struct PairCtx {
+  float rx, ry, rz, e_wca, e_het;
+  uint32_t im1, het;
+};
+__device__ __forceinline__ PairCtx pair_ctx(const KParams &p, float4 me) {
+  PairCtx c;
+  const uint32_t w = wbits(me);
+  c.rx = me.x;
+  c.ry = me.y;
+  c.rz = me.z;
+  c.im1 = (w & ID_MASK) - 1u;
+  c.het = w & W_HET;
+  c.e_wca = p.eps_wca;
+  c.e_het = p.eps_het;
+  asm("" : "+v"(c.im1), "+v"(c.e_wca), "+v"(c.e_het));
+  return c;
+}
+
+// pair_full with the bead's operands from a PairCtx: the same operations in
+// the same order, so the same bits.
+template <int MODE>
+__device__ __forceinline__ void pair_term(const KParams &p, const PairCtx &c, float4 o, float &fx, float &fy,
+                                          float &fz) {
+  const uint32_t wj = wbits(o);
+  const float dx = c.rx - o.x, dy = c.ry - o.y, dz = c.rz - o.z;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  const bool nb = (wj & ID_MASK) - c.im1 > 2u;
+  const bool hhi = (wj & c.het) != 0u;
+  float fac;
+  if constexpr (MODE == MODE_CCP) {
+    const float u = fast_rcp(d2);
+    fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
+    fac = (nb & (d2 <= (hhi ? p.aco2 : p.rco2))) ? fac : 0.0f;
+  } else if constexpr (MODE == MODE_ICG) {
+    const float d = sqrtf(d2);
+    fac = 128.0f * (3.0f * p.rco - 3.0f * d);
+    fac = (nb & (d <= p.rco)) ? fac : 0.0f;
+  } else {
+    const float u = fast_rcp(d2);
+    const float s6 = u * u * u;
+    const float base = s6 * fmaf(48.0f, s6, -24.0f) * u;
+    const bool wca = d2 < p.rm2;
+    const bool on = nb & (wca | (hhi & (d2 < p.rc_lj2)));
+    fac = on ? (wca ? c.e_wca : c.e_het) * base : 0.0f;
   }
   fx += fac * dx;
   fy += fac * dy;
@@ -737,56 +798,33 @@ template <int MODE> __device__ __forceinline__ float trim_radius2(const KParams 
 
 // Squared gaps along one axis between a bead at offset l from its cell's low
 // edge and the cells k + 1 steps below (lo[k]) and above (hi[k]) its own.
+// A cell outside the grid (c is the bead's cell on this axis) gets +inf, so
+// it never passes a distance test and needs no bounds check of its own.
 template <int R> struct AxisGaps {
   float lo[R], hi[R];
 };
-template <int R> __device__ __forceinline__ AxisGaps<R> axis_gaps(float l, float csl) {
+template <int R> __device__ __forceinline__ AxisGaps<R> axis_gaps(float l, float csl, int c, int cps) {
   AxisGaps<R> G;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const float a = fmaxf(l + (float)k * csl, 0.0f), b = fmaxf((float)(k + 1) * csl - l, 0.0f);
-    G.lo[k] = a * a;
-    G.hi[k] = b * b;
+    G.lo[k] = c - (k + 1) >= 0 ? a * a : INFINITY;
+    G.hi[k] = c + (k + 1) < cps ? b * b : INFINITY;
   }
   return G;
 }
-// squared gap to the cell d in [-R, R] steps away (0 for the bead's own)
-template <int R> __device__ __forceinline__ float gap2(const AxisGaps<R> &G, int d) {
-  float v = 0.0f;
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    v = d == -(k + 1) ? G.lo[k] : v;
-    v = d == k + 1 ? G.hi[k] : v;
-  }
-  return v;
+// The same squared gap for one offset d per lane: 0 for d = 0, the gap to
+// the cell d steps away otherwise, +inf outside the grid.
+__device__ __forceinline__ float axis_gap2(float l, float csl, int c, int d, int cps) {
+  const float a = d < 0 ? l + (float)(-d - 1) * csl : (float)d * csl - l;
+  const float g = d == 0 ? 0.0f : fmaxf(a, 0.0f);
+  return (unsigned)(c + d) < (unsigned)cps ? g * g : INFINITY;
 }
-
-// Cells [c0, c1) of row (cy + dy, cz + dz) that can hold a bead within
-// sqrt(rc2) of the bead: empty when the row's y-z slab is farther, else the
-// cells of the row, within reach R, whose x gap is at most the remaining
-// sqrt(rc2 - gap_y^2 - gap_z^2). The gaps grow with the distance in cells,
-// so the kept cells are a contiguous run around cx.
-template <int R>
-__device__ __forceinline__ void trimmed_row(const GridGeom &g, int cx, int cy, int cz, const AxisGaps<R> &X,
-                                            const AxisGaps<R> &Y, const AxisGaps<R> &Z, int dy, int dz, float rc2,
-                                            bool want, int &c0, int &c1) {
-  // branch-free: every lane evaluates the row and an empty one selects 0
-  const int yy = cy + dy, zz = cz + dz;
-  const float h2 = rc2 - gap2<R>(Y, dy) - gap2<R>(Z, dz);
-  const bool ok = want && (unsigned)yy < (unsigned)g.cps && (unsigned)zz < (unsigned)g.cps && h2 >= 0.0f;
-  int xl = cx, xh = cx;
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    xl -= X.lo[k] <= h2 ? 1 : 0;
-    xh += X.hi[k] <= h2 ? 1 : 0;
-  }
-  xl = max(xl, 0);
-  xh = min(xh, g.cps - 1);
-  const int row = g.coff + (zz * g.cps + yy) * g.cps;
-  c0 = ok ? row + xl : 0;
-  c1 = ok ? row + xh + 1 : 0;
+// v of lane k of the caller's eight-lane group (gb4: byte address of the
+// group's lane 0 for ds_bpermute)
+__device__ __forceinline__ float group_lane(float v, int gb4, int k) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(gb4 + (k << 2), __float_as_int(v)));
 }
-
 
 // Reach 2: the 25 neighbour rows (dy, dz) in ring order, the 3 x 3 inner
 // rows first. A cutoff shorter than the cell side never reaches the outer
@@ -823,15 +861,25 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   const int cx = cell_coord(rx, p.gf, outside), cy = cell_coord(ry, p.gf, outside),
             cz = cell_coord(rz, p.gf, outside);
   const float rc2 = trim_radius2<MODE>(p, w);
-  const AxisGaps<R> GX = axis_gaps<R>(rx - (float)(cx - p.gf.half) * p.gf.csl, p.gf.csl),
-                    GY = axis_gaps<R>(ry - (float)(cy - p.gf.half) * p.gf.csl, p.gf.csl),
-                    GZ = axis_gaps<R>(rz - (float)(cz - p.gf.half) * p.gf.csl, p.gf.csl);
+  const float csl = p.gf.csl;
+  const int cps = p.gf.cps;
+  const AxisGaps<R> GX = axis_gaps<R>(rx - (float)(cx - p.gf.half) * csl, csl, cx, cps);
+  // y and z: lane sub of the bead's group holds the squared gaps to the
+  // cells d = sub - R steps away; each row takes its two from those lanes
+  const int d = sub - R, gb4 = (lane & ~(C_LANES - 1)) << 2;
+  const float gyl = axis_gap2(ry - (float)(cy - p.gf.half) * csl, csl, cy, d, cps),
+              gzl = axis_gap2(rz - (float)(cz - p.gf.half) * csl, csl, cz, d, cps);
+  // cell (cx, cy + iy - R, cz + iz - R) is base0 + (iz cps + iy) cps; the
+  // products take the 24-bit multiplier: grids are set up with cps^3 < 2^31,
+  // so cps and iz cps + iy stay below 2^24
+  const int base0 = p.gf.coff + ((cz - R) * cps + (cy - R)) * cps + cx;
 
   // Reach 2: rows 16.. (the outer ring only) are skipped by a wavefront in
   // which no bead's cutoff reaches a cell two steps away in y or z.
   bool any_outer = true;
   if constexpr (R == 2) {
-    const float near2 = fminf(fminf(GY.lo[1], GY.hi[1]), fminf(GZ.lo[1], GZ.hi[1]));
+    const float m2 = fminf(gyl, gzl);
+    const float near2 = fminf(group_lane(m2, gb4, 0), group_lane(m2, gb4, 2 * R));
     any_outer = __ballot(active && rc2 >= near2) != 0ull;
   }
   // rows sub, sub + 8, ... (reach 2: in ring order): trimmed slot ranges and
@@ -842,20 +890,37 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     lo[m] = 0;
     len[m] = 0;
     if (R == 2 && m >= 2 && !any_outer) continue;
+    // Row (dy, dz) = (iy - R, iz - R): its cells [c0, c1) that can hold a
+    // bead within sqrt(rc2), empty when the row's y-z slab is farther (or
+    // outside the grid), else the run around cx whose x gaps fit in the rest,
+    // rc2 - gap_y^2 - gap_z^2. Branch-free: an empty row selects [0, 0).
     const int r = sub + m * C_LANES;
-    int dy = r % W - R, dz = r / W - R;
+    int iy = r % W, iz = r / W;
     if constexpr (R == 2) {
       const uint32_t v = (uint32_t)(RING2_WORDS[m] >> (6 * sub)) & 63u;
-      dy = (int)(v & 7u) - 2;
-      dz = (int)(v >> 3) - 2;
+      iy = (int)(v & 7u);
+      iz = (int)(v >> 3);
     }
-    int c0, c1;
-    trimmed_row<R>(p.gf, cx, cy, cz, GX, GY, GZ, dy, dz, rc2, active && r < NROW, c0, c1);
-    lo[m] = q.fcstart[c0];
-    len[m] = q.fcstart[c1] - lo[m];
+    const float h2 = rc2 - group_lane(gyl, gb4, iy) - group_lane(gzl, gb4, iz);
+    const bool ok = active && r < NROW && h2 >= 0.0f;
+    uint32_t t = __umul24((uint32_t)iz, (uint32_t)cps) + (uint32_t)iy;
+    asm("" : "+v"(t));
+    const int b = (int)__umul24(t, (uint32_t)cps) + base0;
+    int c0 = b, c1 = b + 1;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      c0 -= GX.lo[k] <= h2 ? 1 : 0;
+      c1 += GX.hi[k] <= h2 ? 1 : 0;
+    }
+    // byte offsets of the row's start entries; hidden from the compiler so
+    // that the loads keep the scalar base and a 32-bit offset register
+    uint32_t o0 = ok ? (uint32_t)c0 << 2 : 0u, o1 = ok ? (uint32_t)c1 << 2 : 0u;
+    asm("" : "+v"(o0), "+v"(o1));
+    lo[m] = at_byte(q.fcstart, o0);
+    len[m] = at_byte(q.fcstart, o1) - lo[m];
     tot += len[m];
     if (!BATCH && MODE != MODE_ICG && p.n_l > 0 && (w & W_LAD)) {
-      const uint32_t llo = q.lstart[c0], lhi = q.lstart[c1];
+      const uint32_t llo = at_byte(q.lstart, o0), lhi = at_byte(q.lstart, o1);
       for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
     }
   }
@@ -869,10 +934,14 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   const uint32_t total = __shfl(incl, (lane & ~(C_LANES - 1)) + C_LANES - 1);
   uint32_t off = incl - tot;
   const bool listed = total <= (uint32_t)C_LIST;
+  // the list holds byte offsets (slot x 16) into the ordered positions, so a
+  // gather needs no address arithmetic
   if (listed) {
 #pragma unroll
-    for (int m = 0; m < RPL; ++m)
-      for (uint32_t k = 0; k < len[m]; ++k) list[off++] = lo[m] + k;
+    for (int m = 0; m < RPL; ++m) {
+      uint32_t b = lo[m] << 4;
+      for (uint32_t k = 0; k < len[m]; ++k, b += 16u) list[off++] = b;
+    }
   }
   // the list is read by the lanes of the same wavefront only: complete the
   // LDS writes and order them before the reads, without a block barrier
@@ -880,31 +949,32 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
+  const PairCtx pc = pair_ctx(p, me);
   if (listed) {
     // four gathers in flight per lane; a short tail is padded with the bead's
     // own slot, whose term is exactly zero (|i - j| <= 1 is excluded)
     uint32_t n = sub;
     for (; n + 3 * C_LANES < total; n += 4 * C_LANES) {
-      const float4 a = q.fspos[list[n]], b = q.fspos[list[n + C_LANES]], c = q.fspos[list[n + 2 * C_LANES]],
-                   d = q.fspos[list[n + 3 * C_LANES]];
-      pair_full<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
-      pair_full<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
-      pair_full<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
-      pair_full<MODE>(p, rx, ry, rz, w, d, fx, fy, fz);
+      const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, list[n + C_LANES]),
+                   c = at_byte(q.fspos, list[n + 2 * C_LANES]), d = at_byte(q.fspos, list[n + 3 * C_LANES]);
+      pair_term<MODE>(p, pc, a, fx, fy, fz);
+      pair_term<MODE>(p, pc, b, fx, fy, fz);
+      pair_term<MODE>(p, pc, c, fx, fy, fz);
+      pair_term<MODE>(p, pc, d, fx, fy, fz);
     }
     if (n < total) {
-      const uint32_t jb = n + C_LANES < total ? list[n + C_LANES] : self;
-      const uint32_t jc = n + 2 * C_LANES < total ? list[n + 2 * C_LANES] : self;
-      const float4 a = q.fspos[list[n]], b = q.fspos[jb], c = q.fspos[jc];
-      pair_full<MODE>(p, rx, ry, rz, w, a, fx, fy, fz);
-      pair_full<MODE>(p, rx, ry, rz, w, b, fx, fy, fz);
-      pair_full<MODE>(p, rx, ry, rz, w, c, fx, fy, fz);
+      const uint32_t jb = n + C_LANES < total ? list[n + C_LANES] : self << 4;
+      const uint32_t jc = n + 2 * C_LANES < total ? list[n + 2 * C_LANES] : self << 4;
+      const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, jb), c = at_byte(q.fspos, jc);
+      pair_term<MODE>(p, pc, a, fx, fy, fz);
+      pair_term<MODE>(p, pc, b, fx, fy, fz);
+      pair_term<MODE>(p, pc, c, fx, fy, fz);
     }
   } else {
     // crowded neighbourhood: each lane scans its own rows
 #pragma unroll
     for (int m = 0; m < RPL; ++m)
-      for (uint32_t k = 0; k < len[m]; ++k) pair_full<MODE>(p, rx, ry, rz, w, q.fspos[lo[m] + k], fx, fy, fz);
+      for (uint32_t k = 0; k < len[m]; ++k) pair_term<MODE>(p, pc, q.fspos[lo[m] + k], fx, fy, fz);
   }
 #pragma unroll
   for (int o = C_LANES / 2; o > 0; o >>= 1) {
