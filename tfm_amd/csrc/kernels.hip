@@ -820,6 +820,18 @@ __device__ __forceinline__ float axis_gap2(float l, float csl, int c, int d, int
   const float g = d == 0 ? 0.0f : fmaxf(a, 0.0f);
   return (unsigned)(c + d) < (unsigned)cps ? g * g : INFINITY;
 }
+// v of another lane through a DPP move: quad_perm [1,0,3,2] (lane ^ 1),
+// [2,3,0,1] (lane ^ 2), the half-row mirror (lane 7 - i within eight) and
+// row shifts right by 1, 2, 4 (lane i - n within its 16-lane row). Lanes
+// outside BANKS (4-lane groups of each row), or with no source lane, get 0.
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_MIRROR8 = 0x141, DPP_SHR1 = 0x111, DPP_SHR2 = 0x112,
+              DPP_SHR4 = 0x114;
+template <int CTRL, int BANKS = 0xF> __device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, BANKS, true);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float((int)dpp_u<CTRL>(__float_as_uint(v)));
+}
 // v of lane k of the caller's eight-lane group (gb4: byte address of the
 // group's lane 0 for ds_bpermute)
 __device__ __forceinline__ float group_lane(float v, int gb4, int k) {
@@ -925,13 +937,21 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     }
   }
   // exclusive prefix of the row lengths over the bead's eight lanes
+  // (row shifts by 1, 2, 4 within each 16-lane row, a lane taking only
+  // values of its own group: the shift by 4 reaches lanes 4-7 of each group,
+  // banks 1 and 3), and the group's total by the butterfly of the force sum
   uint32_t incl = tot;
-#pragma unroll
-  for (int o = 1; o < C_LANES; o <<= 1) {
-    const uint32_t v = __shfl_up(incl, o, C_LANES);
-    if (sub >= o) incl += v;
+  {
+    const uint32_t v1 = dpp_u<DPP_SHR1>(incl);
+    incl += sub >= 1 ? v1 : 0u;
+    const uint32_t v2 = dpp_u<DPP_SHR2>(incl);
+    incl += sub >= 2 ? v2 : 0u;
+    incl += dpp_u<DPP_SHR4, 0xA>(incl);
   }
-  const uint32_t total = __shfl(incl, (lane & ~(C_LANES - 1)) + C_LANES - 1);
+  uint32_t total = tot;
+  total += dpp_u<DPP_XOR1>(total);
+  total += dpp_u<DPP_XOR2>(total);
+  total += dpp_u<DPP_MIRROR8>(total);
   uint32_t off = incl - tot;
   const bool listed = total <= (uint32_t)C_LIST;
   // the list holds byte offsets (slot x 16) into the ordered positions, so a
@@ -976,12 +996,12 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     for (int m = 0; m < RPL; ++m)
       for (uint32_t k = 0; k < len[m]; ++k) pair_term<MODE>(p, pc, q.fspos[lo[m] + k], fx, fy, fz);
   }
-#pragma unroll
-  for (int o = C_LANES / 2; o > 0; o >>= 1) {
-    fx += __shfl_xor(fx, o);
-    fy += __shfl_xor(fy, o);
-    fz += __shfl_xor(fz, o);
-  }
+  // sum over the bead's eight lanes: quad swaps (lane ^ 1, lane ^ 2), then
+  // the half-row mirror (lane 7 - i), which pairs each quad with the other;
+  // every lane adds the same two values each time, so all end equal
+  fx += dpp_f<DPP_XOR1>(fx); fy += dpp_f<DPP_XOR1>(fy); fz += dpp_f<DPP_XOR1>(fz);
+  fx += dpp_f<DPP_XOR2>(fx); fy += dpp_f<DPP_XOR2>(fy); fz += dpp_f<DPP_XOR2>(fz);
+  fx += dpp_f<DPP_MIRROR8>(fx); fy += dpp_f<DPP_MIRROR8>(fy); fz += dpp_f<DPP_MIRROR8>(fz);
   // every lane's list reads are done before the group's next list is written
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1163,27 +1183,53 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
   const OrderJob &J = second ? jb : ja;
   const int b = second ? bx - ja.g.nblk : bx;
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
+  const int c = b * SCAN_BLOCK + tid;
+  const int cap = J.g.cap, n_c = J.g.n_c;
+  // The ordering's first loads go out before the per-bead terms below, so
+  // that their latency overlaps that work: this cell's count and the totals
+  // of the workgroups before this one (four per lane at once).
+  constexpr int PRE = 4;
+  const uint32_t cnt = (c < n_c) ? J.count[c] : 0u;
+  uint32_t pv[PRE];
+#pragma unroll
+  for (int u = 0; u < PRE; ++u) {
+    const int k = tid + u * SCAN_BLOCK;
+    pv[u] = k < b ? J.bs_read[k] : 0u;
+  }
   // Per-bead terms of the next force pass at one lane per bead, where the
   // force kernels would run them on one lane of eight: the next step's
   // normals (nj.z) and the bonded + wall forces of the positions just
   // ordered (bj.out). The first N / 256 workgroups take them, in bead order,
-  // ahead of the ordering so that their arithmetic covers its first loads.
+  // ahead of the ordering so that their arithmetic covers its first loads
+  // (workgroups of their own beside the ordering ones measured slower).
   if (nj.z)
     for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < nj.n; g += gridDim.x * SCAN_BLOCK) fill_noise(nj, g);
   if (bj.out)
     for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < bj.n; g += gridDim.x * SCAN_BLOCK) fill_bond(bp, bj, g);
-  const int c = b * SCAN_BLOCK + tid;
-  const int cap = J.g.cap, n_c = J.g.n_c;
-  uint32_t part = 0;
-  for (int k = tid; k < b; k += SCAN_BLOCK) part += J.bs_read[k];
+  uint32_t part = (pv[0] + pv[1]) + (pv[2] + pv[3]);
+  for (int k0 = tid + PRE * SCAN_BLOCK; k0 < b; k0 += PRE * SCAN_BLOCK) {
+#pragma unroll
+    for (int u = 0; u < PRE; ++u) {
+      const int k = k0 + u * SCAN_BLOCK;
+      pv[u] = k < b ? J.bs_read[k] : 0u;
+    }
+    part += (pv[0] + pv[1]) + (pv[2] + pv[3]);
+  }
   part = wave_sum(part);
   if (lane == 0) red[wid] = part;
-  const uint32_t cnt = (c < n_c) ? J.count[c] : 0u;
   uint32_t m = cnt;
   if (m > (uint32_t)cap) {
     atomicAdd(flags + FLAG_OVERFLOW, 1u);
     m = (uint32_t)cap;
   }
+  // the cell's first entries, loaded together now so that their latency
+  // overlaps the scans and the barrier below
+  const float4 *bk = J.bucket + (size_t)c * cap;
+  constexpr uint32_t EARLY = 4;
+  float4 ev[EARLY];
+#pragma unroll
+  for (uint32_t k = 0; k < EARLY; ++k)
+    if (k < m) ev[k] = bk[k];
   const uint32_t incl = wave_incl_scan(cnt);
   const uint32_t mincl = wave_incl_scan(m);
   if (lane == 63) {
@@ -1212,17 +1258,19 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
     J.bs_clear[b] = 0u;
     if (b == 0 && !second && step_inc) *step_inc += 1ull;
   }
-  const float4 *bk = J.bucket + (size_t)c * cap;
   if (mtotal <= (uint32_t)ORDER_LDS) {
     s_loff[tid] = moff;
     s_start[tid] = start;
     if (tid == SCAN_BLOCK - 1) s_loff[SCAN_BLOCK] = moff + m;
-    for (uint32_t k = 0; k < m; ++k) {
-      const float4 v = bk[k];
+    auto put = [&](uint32_t k, float4 v) {
       s_pos[moff + k] = v;
       s_id[moff + k] = __float_as_uint(v.w) & ID_MASK;
       s_cell[moff + k] = (uint8_t)tid;
-    }
+    };
+#pragma unroll
+    for (uint32_t k = 0; k < EARLY; ++k)
+      if (k < m) put(k, ev[k]);
+    for (uint32_t k = EARLY; k < m; ++k) put(k, bk[k]);
     __syncthreads();
     for (uint32_t e = tid; e < mtotal; e += SCAN_BLOCK) {
       const uint32_t l = s_cell[e], id = s_id[e];
