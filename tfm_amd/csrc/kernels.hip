@@ -971,10 +971,15 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
 
   const PairCtx pc = pair_ctx(p, me);
   if (listed) {
-    // four gathers in flight per lane; a short tail is padded with the bead's
-    // own slot, whose term is exactly zero (|i - j| <= 1 is excluded)
-    uint32_t n = sub;
-    for (; n + 3 * C_LANES < total; n += 4 * C_LANES) {
+    // Rounds of four gathers in flight per lane (entries n, n + 8, n + 16,
+    // n + 24): full ones while the bead's list holds 32 more, then one tail
+    // round, four wide if some bead of the wavefront has more than 24 left
+    // and three otherwise. The tail is padded with the bead's own slot, whose
+    // term is exactly zero (|i - j| <= 1 is excluded), so each lane adds the
+    // same entries in the same order whatever the width.
+    uint32_t base = 0;
+    for (; base + 4 * C_LANES <= total; base += 4 * C_LANES) {
+      const uint32_t n = base + sub;
       const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, list[n + C_LANES]),
                    c = at_byte(q.fspos, list[n + 2 * C_LANES]), d = at_byte(q.fspos, list[n + 3 * C_LANES]);
       pair_term<MODE>(p, pc, a, fx, fy, fz);
@@ -982,13 +987,18 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
       pair_term<MODE>(p, pc, c, fx, fy, fz);
       pair_term<MODE>(p, pc, d, fx, fy, fz);
     }
+    const uint32_t rem = total - base, n = base + sub, s4 = self << 4;
+    const bool wide = __ballot(rem > 3 * C_LANES) != 0ull;
     if (n < total) {
-      const uint32_t jb = n + C_LANES < total ? list[n + C_LANES] : self << 4;
-      const uint32_t jc = n + 2 * C_LANES < total ? list[n + 2 * C_LANES] : self << 4;
+      const uint32_t jb = n + C_LANES < total ? list[n + C_LANES] : s4;
+      const uint32_t jc = n + 2 * C_LANES < total ? list[n + 2 * C_LANES] : s4;
       const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, jb), c = at_byte(q.fspos, jc);
+      float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (wide) d = at_byte(q.fspos, n + 3 * C_LANES < total ? list[n + 3 * C_LANES] : s4);
       pair_term<MODE>(p, pc, a, fx, fy, fz);
       pair_term<MODE>(p, pc, b, fx, fy, fz);
       pair_term<MODE>(p, pc, c, fx, fy, fz);
+      if (wide) pair_term<MODE>(p, pc, d, fx, fy, fz);
     }
   } else {
     // crowded neighbourhood: each lane scans its own rows
