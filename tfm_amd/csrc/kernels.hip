@@ -894,13 +894,14 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     const float near2 = fminf(group_lane(m2, gb4, 0), group_lane(m2, gb4, 2 * R));
     any_outer = __ballot(active && rc2 >= near2) != 0ull;
   }
-  // rows sub, sub + 8, ... (reach 2: in ring order): trimmed slot ranges and
-  // the LBS sites of LAD beads
-  uint32_t lo[RPL], len[RPL], tot = 0;
+  // rows sub, sub + 8, ... (reach 2: in ring order): the byte offsets of
+  // their trimmed slot ranges first, then the start entries of all of them
+  // loaded together, so that a lane waits once for all its rows
+  uint32_t o0[RPL], o1[RPL];
 #pragma unroll
   for (int m = 0; m < RPL; ++m) {
-    lo[m] = 0;
-    len[m] = 0;
+    o0[m] = 0u;
+    o1[m] = 0u;
     if (R == 2 && m >= 2 && !any_outer) continue;
     // Row (dy, dz) = (iy - R, iz - R): its cells [c0, c1) that can hold a
     // bead within sqrt(rc2), empty when the row's y-z slab is farther (or
@@ -924,15 +925,29 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
       c0 -= GX.lo[k] <= h2 ? 1 : 0;
       c1 += GX.hi[k] <= h2 ? 1 : 0;
     }
-    // byte offsets of the row's start entries; hidden from the compiler so
-    // that the loads keep the scalar base and a 32-bit offset register
-    uint32_t o0 = ok ? (uint32_t)c0 << 2 : 0u, o1 = ok ? (uint32_t)c1 << 2 : 0u;
-    asm("" : "+v"(o0), "+v"(o1));
-    lo[m] = at_byte(q.fcstart, o0);
-    len[m] = at_byte(q.fcstart, o1) - lo[m];
+    // hidden from the compiler so that the loads keep the scalar base and a
+    // 32-bit offset register
+    uint32_t a0 = ok ? (uint32_t)c0 << 2 : 0u, a1 = ok ? (uint32_t)c1 << 2 : 0u;
+    asm("" : "+v"(a0), "+v"(a1));
+    o0[m] = a0;
+    o1[m] = a1;
+  }
+  uint32_t lo[RPL], len[RPL], tot = 0;
+#pragma unroll
+  for (int m = 0; m < RPL; ++m) {
+    lo[m] = at_byte(q.fcstart, o0[m]);
+    len[m] = at_byte(q.fcstart, o1[m]);
+  }
+#pragma unroll
+  for (int m = 0; m < RPL; ++m) {
+    len[m] -= lo[m];
     tot += len[m];
-    if (!BATCH && MODE != MODE_ICG && p.n_l > 0 && (w & W_LAD)) {
-      const uint32_t llo = at_byte(q.lstart, o0), lhi = at_byte(q.lstart, o1);
+  }
+  // the LBS sites of LAD beads, row by row
+  if (!BATCH && MODE != MODE_ICG && p.n_l > 0 && (w & W_LAD)) {
+#pragma unroll
+    for (int m = 0; m < RPL; ++m) {
+      const uint32_t llo = at_byte(q.lstart, o0[m]), lhi = at_byte(q.lstart, o1[m]);
       for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
     }
   }
