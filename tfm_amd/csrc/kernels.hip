@@ -986,13 +986,22 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
 
   const PairCtx pc = pair_ctx(p, me);
   if (listed) {
-    // Rounds of four gathers in flight per lane (entries n, n + 8, n + 16,
-    // n + 24): full ones while the bead's list holds 32 more, then one tail
+    // Rounds of gathers in flight per lane (entries n, n + 8, n + 16, ...):
+    // full ones of four while the bead's list holds 32 more, then one tail
     // round, four wide if some bead of the wavefront has more than 24 left
     // and three otherwise. The tail is padded with the bead's own slot, whose
     // term is exactly zero (|i - j| <= 1 is excluded), so each lane adds the
     // same entries in the same order whatever the width.
     uint32_t base = 0;
+    // long lists (het beads): eight gathers in flight
+    for (; base + 8 * C_LANES <= total; base += 8 * C_LANES) {
+      const uint32_t n = base + sub;
+      float4 e[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) e[k] = at_byte(q.fspos, list[n + k * C_LANES]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) pair_term<MODE>(p, pc, e[k], fx, fy, fz);
+    }
     for (; base + 4 * C_LANES <= total; base += 4 * C_LANES) {
       const uint32_t n = base + sub;
       const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, list[n + C_LANES]),
