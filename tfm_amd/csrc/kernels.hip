@@ -270,27 +270,29 @@ __device__ __forceinline__ void lbs_force(const KParams &p, float rx, float ry, 
 
 // Bonds and bending on bead i (calc_bf, chrsim.cu:13-76); bond existence
 // comes from the W_B* bits, positions from the bead-ordered array.
-__device__ __forceinline__ void bonded_force(const KParams &p, const float4 *__restrict__ pos, uint32_t i,
-                                             uint32_t w, float rx, float ry, float rz, float &fx, float &fy,
-                                             float &fz) {
+// nb(k): the position of bead i + k (k = -2, -1, 1, 2), read only where the
+// chain has that neighbour
+template <class NB>
+__device__ __forceinline__ void bonded_terms(const KParams &p, NB nb, uint32_t w, float rx, float ry, float rz,
+                                             float &fx, float &fy, float &fz) {
   float v0x = 0, v0y = 0, v0z = 0, v1x = 0, v1y = 0, v1z = 0, v2x = 0, v2y = 0, v2z = 0, v3x = 0, v3y = 0, v3z = 0;
   float il0 = 0, il1 = 0, il2 = 0, il3 = 0;
   if (w & W_B1) {
-    const float4 a = pos[i - 1];
+    const float4 a = nb(-1);
     v1x = rx - a.x; v1y = ry - a.y; v1z = rz - a.z;
     il1 = rsqrtf(v1x * v1x + v1y * v1y + v1z * v1z);
     if (w & W_B0) {
-      const float4 b = pos[i - 2];
+      const float4 b = nb(-2);
       v0x = a.x - b.x; v0y = a.y - b.y; v0z = a.z - b.z;
       il0 = rsqrtf(v0x * v0x + v0y * v0y + v0z * v0z);
     }
   }
   if (w & W_B2) {
-    const float4 a = pos[i + 1];
+    const float4 a = nb(1);
     v2x = a.x - rx; v2y = a.y - ry; v2z = a.z - rz;
     il2 = rsqrtf(v2x * v2x + v2y * v2y + v2z * v2z);
     if (w & W_B3) {
-      const float4 b = pos[i + 2];
+      const float4 b = nb(2);
       v3x = b.x - a.x; v3y = b.y - a.y; v3z = b.z - a.z;
       il3 = rsqrtf(v3x * v3x + v3y * v3y + v3z * v3z);
     }
@@ -336,6 +338,12 @@ __device__ __forceinline__ void bonded_force(const KParams &p, const float4 *__r
     fy += c0 * v0y + c1 * v1y + c2 * v2y + c3 * v3y;
     fz += c0 * v0z + c1 * v1z + c2 * v2z + c3 * v3z;
   }
+}
+
+__device__ __forceinline__ void bonded_force(const KParams &p, const float4 *__restrict__ pos, uint32_t i,
+                                             uint32_t w, float rx, float ry, float rz, float &fx, float &fy,
+                                             float &fz) {
+  bonded_terms(p, [&](int k) { return pos[i + k]; }, w, rx, ry, rz, fx, fy, fz);
 }
 
 // Confinement (calc_cf<DST>, chrsim.cu:91-144, sphere + opening + bleb;
@@ -609,17 +617,33 @@ __device__ __forceinline__ void fill_noise(const NoiseJob &nj, uint32_t g) {
 }
 
 // Bonded + wall force of bead g for the next force pass (one lane per bead).
-__device__ __forceinline__ void fill_bond(KParams p, const BondJob &bj, uint32_t g) {
+// The bead and its four chain neighbours (indices clamped to [0, n), read
+// whether or not the chain has them), loaded together so that they share one
+// wait; bonded_terms uses only the ones it needs.
+struct BondNbrs {
+  float4 me, m2, m1, p1, p2;
+};
+__device__ __forceinline__ BondNbrs load_bond_nbrs(const BondJob &bj, uint32_t g) {
+  BondNbrs b;
+  b.me = bj.pos[g];
+  b.m2 = bj.pos[g >= 2u ? g - 2u : g];
+  b.m1 = bj.pos[g >= 1u ? g - 1u : g];
+  b.p1 = bj.pos[g + 1u < bj.n ? g + 1u : g];
+  b.p2 = bj.pos[g + 2u < bj.n ? g + 2u : g];
+  return b;
+}
+__device__ __forceinline__ void fill_bond(KParams p, const BondJob &bj, uint32_t g, const BondNbrs &b) {
   if (bj.sys) {
     const SysParams &S = bj.sys[system_of(bj.sys_slot, bj.n_sys, g)];
     p.R_n = S.R_n; p.R_o = S.R_o; p.R_b = S.R_b;
     p.noc = S.noc; p.boc = S.boc; p.nod = S.nod; p.bod = S.bod;
     p.eps_het = S.eps_het; p.eps_lam = S.eps_lam;
   }
-  const float4 me = bj.pos[g];
+  const float4 me = b.me;
   const uint32_t w = wbits(me);
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  bonded_force(p, bj.pos, g, w, me.x, me.y, me.z, fx, fy, fz);
+  bonded_terms(p, [&](int k) { return k == -2 ? b.m2 : k == -1 ? b.m1 : k == 1 ? b.p1 : b.p2; }, w, me.x, me.y,
+               me.z, fx, fy, fz);
   if (bj.mode == MODE_LJ) wall_force<MODE_LJ>(p, me.x, me.y, me.z, w, fx, fy, fz);
   else if (bj.mode == MODE_ICG) wall_force<MODE_ICG>(p, me.x, me.y, me.z, w, fx, fy, fz);
   else wall_force<MODE_CCP>(p, me.x, me.y, me.z, w, fx, fy, fz);
@@ -1236,10 +1260,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
   // ordered (bj.out). The first N / 256 workgroups take them, in bead order,
   // ahead of the ordering so that their arithmetic covers its first loads
   // (workgroups of their own beside the ordering ones measured slower).
-  if (nj.z)
-    for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < nj.n; g += gridDim.x * SCAN_BLOCK) fill_noise(nj, g);
-  if (bj.out)
-    for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < bj.n; g += gridDim.x * SCAN_BLOCK) fill_bond(bp, bj, g);
+  // A bead's positions for its bonded terms are loaded before its normals
+  // are drawn, so that the two overlap.
+  {
+    const uint32_t n_terms = max(nj.z ? nj.n : 0u, bj.out ? bj.n : 0u);
+    for (uint32_t g = blockIdx.x * SCAN_BLOCK + tid; g < n_terms; g += gridDim.x * SCAN_BLOCK) {
+      const bool bond = bj.out && g < bj.n;
+      BondNbrs nbr;
+      if (bond) nbr = load_bond_nbrs(bj, g);
+      if (nj.z && g < nj.n) fill_noise(nj, g);
+      if (bond) fill_bond(bp, bj, g, nbr);
+    }
+  }
   uint32_t part = (pv[0] + pv[1]) + (pv[2] + pv[3]);
   for (int k0 = tid + PRE * SCAN_BLOCK; k0 < b; k0 += PRE * SCAN_BLOCK) {
 #pragma unroll
@@ -1340,7 +1372,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
 __global__ __launch_bounds__(BEAD_BLOCK) void k_bead_terms(NoiseJob nj, KParams p, BondJob bj) {
   const uint32_t g = blockIdx.x * BEAD_BLOCK + threadIdx.x;
   if (nj.z && g < nj.n) fill_noise(nj, g);
-  if (bj.out && g < bj.n) fill_bond(p, bj, g);
+  if (bj.out && g < bj.n) fill_bond(p, bj, g, load_bond_nbrs(bj, g));
 }
 
 // Contact counts on the fine grid: pairs i < j with d < rc within
