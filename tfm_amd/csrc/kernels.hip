@@ -86,20 +86,36 @@ __device__ __forceinline__ uint32_t noise3(uint32_t k0, uint32_t k1, uint32_t be
 }
 
 // ---------------------------------------------------------------- wave helpers
+// v of another lane through a DPP move: quad_perm [1,0,3,2] (lane ^ 1),
+// [2,3,0,1] (lane ^ 2), the half-row mirror (lane 7 - i within eight), row
+// shifts right by n (lane i - n within its 16-lane row) and the row
+// broadcasts (lane 15 of each row into the next row, lane 31 into rows 2
+// and 3). Lanes outside ROWS (16-lane rows) or BANKS (4-lane groups of each
+// row), or with no source lane, get 0.
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_MIRROR8 = 0x141, DPP_SHR1 = 0x111, DPP_SHR2 = 0x112,
+              DPP_SHR4 = 0x114, DPP_SHR8 = 0x118, DPP_BCAST15 = 0x142, DPP_BCAST31 = 0x143;
+template <int CTRL, int BANKS = 0xF, int ROWS = 0xF> __device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, BANKS, true);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float((int)dpp_u<CTRL>(__float_as_uint(v)));
+}
+
+// Inclusive prefix over the 64 lanes (all active): within each 16-lane row
+// by shifts of 1, 2, 4, 8, then the rows' totals carried by the broadcasts.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const int lane = __lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = __shfl_up(v, o);
-    if (lane >= o) v += t;
-  }
+  v += dpp_u<DPP_SHR1>(v);
+  v += dpp_u<DPP_SHR2>(v);
+  v += dpp_u<DPP_SHR4>(v);
+  v += dpp_u<DPP_SHR8>(v);
+  v += dpp_u<DPP_BCAST15, 0xF, 0xA>(v);
+  v += dpp_u<DPP_BCAST31, 0xF, 0xC>(v);
   return v;
 }
 
+// sum over the 64 lanes (all active), the same in every lane
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), 63);
 }
 
 // atomicAdd(base[key], 1) for every active lane, one atomic per distinct key
@@ -154,9 +170,10 @@ __device__ __forceinline__ void bin_into(const GridGeom &g, uint32_t *count, uin
   const uint32_t key = (uint32_t)(g.coff + (gz * g.cps + gy) * g.cps + gx);
   // beads of one wavefront mostly sit in distinct cells: plain atomics
   const uint32_t slot = atomicAdd(count + key, 1u);
-  if (slot < (uint32_t)g.cap) bucket[(size_t)key * g.cap + slot] = v;
+  // the block totals and the flag need no slot: done while it returns
   agg_add_noret(blocksum, key / SCAN_BLOCK);
   if (outside) atomicAdd(flags + FLAG_OUTSIDE, 1u);
+  if (slot < (uint32_t)g.cap) bucket[(size_t)key * g.cap + slot] = v;
 }
 
 // Bin a new bead position into the next fine grid and, for het beads, the
@@ -843,18 +860,6 @@ __device__ __forceinline__ float axis_gap2(float l, float csl, int c, int d, int
   const float a = d < 0 ? l + (float)(-d - 1) * csl : (float)d * csl - l;
   const float g = d == 0 ? 0.0f : fmaxf(a, 0.0f);
   return (unsigned)(c + d) < (unsigned)cps ? g * g : INFINITY;
-}
-// v of another lane through a DPP move: quad_perm [1,0,3,2] (lane ^ 1),
-// [2,3,0,1] (lane ^ 2), the half-row mirror (lane 7 - i within eight) and
-// row shifts right by 1, 2, 4 (lane i - n within its 16-lane row). Lanes
-// outside BANKS (4-lane groups of each row), or with no source lane, get 0.
-constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_MIRROR8 = 0x141, DPP_SHR1 = 0x111, DPP_SHR2 = 0x112,
-              DPP_SHR4 = 0x114;
-template <int CTRL, int BANKS = 0xF> __device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, BANKS, true);
-}
-template <int CTRL> __device__ __forceinline__ float dpp_f(float v) {
-  return __int_as_float((int)dpp_u<CTRL>(__float_as_uint(v)));
 }
 // v of lane k of the caller's eight-lane group (gb4: byte address of the
 // group's lane 0 for ds_bpermute)
