@@ -1,26 +1,32 @@
 // gfx950 kernels of the ccp Brownian-dynamics step and of ccp's analysis.
 //
 // One stochastic-Heun stage is two launches:
-//   k_force<MODE, STAGE>  three lanes per bead, beads in grid order: lane p
-//                         scans the 3 neighbour rows of plane z + p - 1 of
-//                         the all-bead grid (ccp's 27-cell neighbourhood,
-//                         calc_srf, chrsim.cu:277-317) and the LBS sites
-//                         binned on the same geometry; lane 0 adds bonds,
-//                         bending and the wall; two shuffles sum the parts.
-//                         Then the Heun update, and binning of the new
-//                         position into the next grid (wave-aggregated
-//                         atomics on per-cell counters and per-scan-block
-//                         totals, bucket slot). Optionally (fine_cell_size <
-//                         cell_size) the pair sum is split by cutoff: short
-//                         pairs from a fine all-bead grid, the long het-het
+//   k_force_bal<MODE, STAGE>  eight lanes per bead, beads in cell order (het
+//                         beads first within each workgroup): the bead's
+//                         neighbour rows (ccp's 27-cell neighbourhood,
+//                         calc_srf, chrsim.cu:277-317, or 125 half-size
+//                         cells) trimmed to its own cutoff, their slots
+//                         compacted into an LDS list that the eight lanes
+//                         share evenly, the partial forces summed by DPP.
+//                         Lane 0 then adds the bonded + wall force of the
+//                         order pass, runs the Heun update and bins the new
+//                         position into the next grid (atomic count -> bucket
+//                         slot, aggregated per-scan-block totals).
+//   k_force<MODE, STAGE>  the other kernel shapes: one lane per bead, or one
+//                         lane per neighbour plane with shuffles summing the
+//                         parts; optionally (fine_cell_size < cell_size) short
+//                         pairs from a fine all-bead grid and the long het-het
 //                         tail from a coarse grid of het beads only.
 //   k_order               one thread per cell: exclusive scan of the counts
 //                         (block prefix from the per-block totals), then a
-//                         rank-by-bead-index ordering of each cell's bucket,
-//                         so the ordered arrays equal a stable sort by cell
+//                         rank-by-bead-index ordering of each cell's bucket
+//                         entries (positions with the bead index in .w), so
+//                         the ordered positions equal a stable sort by cell
 //                         (the result of ccp's cub::DeviceRadixSort::SortPairs
 //                         over identity values, sugrid.cu:125-133) bit for
-//                         bit, and a gather of the positions into cell order.
+//                         bit. Its first workgroups also compute the next
+//                         force pass's per-bead terms at one lane per bead:
+//                         the next step's normals and the bonded + wall forces.
 // Workgroup indices are remapped so each XCD works on one contiguous run of
 // the grid (its L2 holds that run's neighbour rows).
 // Analysis kernels (chrana.cu): frame observables with FP64 sums, d(s)/P(s)
@@ -821,7 +827,7 @@ __device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q,
 // cutoff (the het-het one for het beads, the short one otherwise); the slots
 // of those cells are compacted into an LDS list, and the eight lanes take
 // every eighth entry. A bead's candidates are then spread evenly over its
-// lanes whatever the row lengths; three butterfly steps sum the parts.
+// lanes whatever the row lengths; three DPP steps sum the parts.
 constexpr int C_LANES = 8;
 constexpr int C_BEADS_PER_BLOCK = (64 / C_LANES) * (BEAD_BLOCK / 64);
 constexpr int C_LIST = 128; // candidate slots per bead held in LDS
@@ -983,7 +989,8 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   // exclusive prefix of the row lengths over the bead's eight lanes
   // (row shifts by 1, 2, 4 within each 16-lane row, a lane taking only
   // values of its own group: the shift by 4 reaches lanes 4-7 of each group,
-  // banks 1 and 3), and the group's total by the butterfly of the force sum
+  // banks 1 and 3), and the group's total by the three DPP steps of the
+  // force sum
   uint32_t incl = tot;
   {
     const uint32_t v1 = dpp_u<DPP_SHR1>(incl);
