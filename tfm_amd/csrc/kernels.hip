@@ -829,7 +829,11 @@ __device__ __forceinline__ void stage_epilogue(const KParams &p, const KPtrs &q,
 // every eighth entry. A bead's candidates are then spread evenly over its
 // lanes whatever the row lengths; three DPP steps sum the parts.
 constexpr int C_LANES = 8;
-constexpr int C_BEADS_PER_BLOCK = (64 / C_LANES) * (BEAD_BLOCK / 64);
+// Workgroups of two wavefronts (16 beads): 256- and 512-lane ones group the
+// types over more beads but schedule coarser, and measured 1.8 % and 9 %
+// slower per step.
+constexpr int BAL_BLOCK = 128;
+constexpr int C_BEADS_PER_BLOCK = (64 / C_LANES) * (BAL_BLOCK / 64);
 constexpr int C_LIST = 128; // candidate slots per bead held in LDS
 static_assert(C_BEADS_PER_BLOCK <= 32, "the type grouping keeps one bit per bead in 32 bits");
 
@@ -1089,13 +1093,13 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
 // writes it to q.fpair; k_integrate (one lane per bead) adds bonds and wall
 // and runs the stage.
 template <int MODE, int STAGE, bool BATCH, int R, bool EPI = true, int ROUNDS = 1>
-__global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
+__global__ __launch_bounds__(BAL_BLOCK) void k_force_bal(KParams p, KPtrs q) {
   static_assert(ROUNDS == 1 || EPI, "the split integration pass takes one round");
   static_assert(ROUNDS >= 1 && ROUNDS <= C_LANES, "a group keeps at most one bead per lane");
   __shared__ uint32_t s_list[C_BEADS_PER_BLOCK][C_LIST];
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const int sub = lane % C_LANES, gw = lane / C_LANES, grp = wave * (64 / C_LANES) + gw;
-  const int wbase = (xcd_block() * (BEAD_BLOCK / 64) + wave) * ((64 / C_LANES) * ROUNDS);
+  const int wbase = (xcd_block() * (BAL_BLOCK / 64) + wave) * ((64 / C_LANES) * ROUNDS);
   if constexpr (ROUNDS == 1) {
     int t = wbase + gw;
     float4 me = t < p.N ? q.fspos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1105,7 +1109,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
       // workgroup's slots are reordered: het slots first, then the rest, each
       // in cell order. Position grp takes the k-th set bit of the het mask;
       // the positions move through LDS.
-      __shared__ uint32_t s_mask[BEAD_BLOCK / 64];
+      __shared__ uint32_t s_mask[BAL_BLOCK / 64];
       __shared__ float4 s_me[C_BEADS_PER_BLOCK];
       const int t0 = wbase - wave * (64 / C_LANES);
       const bool het = t < p.N && (wbits(me) & W_HET) != 0u;
@@ -1118,7 +1122,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_force_bal(KParams p, KPtrs q) {
       __syncthreads();
       uint32_t m = 0;
 #pragma unroll
-      for (int k = 0; k < BEAD_BLOCK / 64; ++k) m |= s_mask[k] << (k * (64 / C_LANES));
+      for (int k = 0; k < BAL_BLOCK / 64; ++k) m |= s_mask[k] << (k * (64 / C_LANES));
       const int nh = __popc(m);
       uint32_t mm = grp < nh ? m : ~m;
       int k = grp < nh ? grp : grp - nh, pos = 0;
@@ -1694,7 +1698,7 @@ static void force_balanced(hipStream_t s, int stage, const KParams &p, const KPt
   const unsigned g = blocks_for(p.N, C_BEADS_PER_BLOCK);
   if (q.fpair) {
     // pair sum with eight lanes per bead, then bonds, wall and the stage with one
-    k_force_bal<MODE, 0, BATCH, R, false><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+    k_force_bal<MODE, 0, BATCH, R, false><<<g, BAL_BLOCK, 0, s>>>(p, q);
     const unsigned gi = blocks_for(p.N, BEAD_BLOCK);
     if (stage == 0) k_integrate<MODE, 0, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
     else if (stage == 1) k_integrate<MODE, 1, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
@@ -1703,28 +1707,28 @@ static void force_balanced(hipStream_t s, int stage, const KParams &p, const KPt
   }
   if (p.rounds == 4) {
     const unsigned g4 = blocks_for(p.N, C_BEADS_PER_BLOCK * 4);
-    if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 4><<<g4, BEAD_BLOCK, 0, s>>>(p, q);
-    else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 4><<<g4, BEAD_BLOCK, 0, s>>>(p, q);
-    else k_force_bal<MODE, 2, BATCH, R, true, 4><<<g4, BEAD_BLOCK, 0, s>>>(p, q);
+    if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 4><<<g4, BAL_BLOCK, 0, s>>>(p, q);
+    else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 4><<<g4, BAL_BLOCK, 0, s>>>(p, q);
+    else k_force_bal<MODE, 2, BATCH, R, true, 4><<<g4, BAL_BLOCK, 0, s>>>(p, q);
     return;
   }
   if (p.rounds == 2) {
     const unsigned g2 = blocks_for(p.N, C_BEADS_PER_BLOCK * 2);
-    if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 2><<<g2, BEAD_BLOCK, 0, s>>>(p, q);
-    else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 2><<<g2, BEAD_BLOCK, 0, s>>>(p, q);
-    else k_force_bal<MODE, 2, BATCH, R, true, 2><<<g2, BEAD_BLOCK, 0, s>>>(p, q);
+    if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 2><<<g2, BAL_BLOCK, 0, s>>>(p, q);
+    else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 2><<<g2, BAL_BLOCK, 0, s>>>(p, q);
+    else k_force_bal<MODE, 2, BATCH, R, true, 2><<<g2, BAL_BLOCK, 0, s>>>(p, q);
     return;
   }
   if (p.rounds == 8) {
     const unsigned g8 = blocks_for(p.N, C_BEADS_PER_BLOCK * 8);
-    if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 8><<<g8, BEAD_BLOCK, 0, s>>>(p, q);
-    else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 8><<<g8, BEAD_BLOCK, 0, s>>>(p, q);
-    else k_force_bal<MODE, 2, BATCH, R, true, 8><<<g8, BEAD_BLOCK, 0, s>>>(p, q);
+    if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 8><<<g8, BAL_BLOCK, 0, s>>>(p, q);
+    else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 8><<<g8, BAL_BLOCK, 0, s>>>(p, q);
+    else k_force_bal<MODE, 2, BATCH, R, true, 8><<<g8, BAL_BLOCK, 0, s>>>(p, q);
     return;
   }
-  if (stage == 0) k_force_bal<MODE, 0, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);
-  else if (stage == 1) k_force_bal<MODE, 1, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);
-  else k_force_bal<MODE, 2, BATCH, R><<<g, BEAD_BLOCK, 0, s>>>(p, q);
+  if (stage == 0) k_force_bal<MODE, 0, BATCH, R><<<g, BAL_BLOCK, 0, s>>>(p, q);
+  else if (stage == 1) k_force_bal<MODE, 1, BATCH, R><<<g, BAL_BLOCK, 0, s>>>(p, q);
+  else k_force_bal<MODE, 2, BATCH, R><<<g, BAL_BLOCK, 0, s>>>(p, q);
 }
 
 // kernel shape: cell reach 1 or 2; one lane per bead, one per plane, or
