@@ -125,25 +125,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 // atomicAdd(base[key], 1) for every active lane, one atomic per distinct key
-// in the wavefront (ballot match loop); returns the lane's slot = old value +
-// rank among the lanes of the same key.
-__device__ __forceinline__ uint32_t agg_add(uint32_t *base, uint32_t key) {
-  const int lane = __lane_id();
-  unsigned long long todo = __ballot(1);
-  uint32_t res = 0;
-  while (todo) {
-    const int leader = __ffsll(todo) - 1;
-    const uint32_t lkey = __shfl(key, leader);
-    const unsigned long long same = __ballot(key == lkey) & todo;
-    uint32_t old = 0;
-    if (lane == leader) old = atomicAdd(base + lkey, (uint32_t)__popcll(same));
-    old = __shfl(old, leader);
-    if ((same >> lane) & 1ull) res = old + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-    todo &= ~same;
-  }
-  return res;
-}
-
+// in the wavefront (ballot match loop)
 __device__ __forceinline__ void agg_add_noret(uint32_t *base, uint32_t key) {
   const int lane = __lane_id();
   unsigned long long todo = __ballot(1);
