@@ -108,6 +108,29 @@ def test_forces_match_oracle(case):
     sim.close()
 
 
+@pytest.mark.parametrize("rounds", [1, 4])
+def test_forces_partial_workgroup(rounds):
+    # N = 1997 leaves the last workgroup of the eight-lane kernel part full
+    # (16 beads per workgroup, 64 with four rounds) and its type grouping
+    # with fewer het beads than slots
+    N = 1997
+    step(f"forces, partial last workgroup, N={N}, bal_rounds={rounds}")
+    R_n = radius_for(N, 0.3)
+    r = relaxed(N, R_n, 11, chain_off=[0, 998, N])
+    r *= 1.0 + 0.3 / R_n
+    rng = np.random.default_rng(12)
+    fl = ((rng.random(N) < 0.3) * po.HET | (rng.random(N) < 0.15) * po.LAD).astype(np.uint8)
+    sim, osys, r64 = make_pair(N, R_n, fl, r, ff=po.FF_LJ, lbs=lbs_sites(R_n, 30, 3), chain_off=[0, 998, N],
+                               eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, bal_rounds=rounds)
+    got = sim.compute_forces(po.FF_LJ).astype(np.float64)
+    want = osys.forces(r64, method=0)
+    w32 = osys.forces(r64, method=0, prec=32)
+    err, err32, scale = np.abs(got - want).max(), np.abs(w32 - want).max(), np.abs(want).max()
+    step(f"  max|f|={scale:.3g} gpu err={err:.3g} oracle fp32 err={err32:.3g}")
+    assert err <= 3 * err32 + 1e-6 * scale
+    sim.close()
+
+
 @pytest.mark.parametrize("ff,dt,tol,reach,lanes,rounds", [
     (po.FF_CCP, 1 / 2048, 1e-3, 1, 3, 1), (po.FF_LJ, 1e-3, 2e-3, 1, 3, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 5, 1),
     (po.FF_CCP, 1 / 2048, 1e-3, 1, 8, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 8, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 8, 4),
