@@ -866,17 +866,18 @@ constexpr int RING2[25][2] = {{0, 0},   {-1, 0}, {1, 0},  {0, -1}, {0, 1},  {-1,
                               {-1, 1},  {1, 1},  {-2, -2}, {-1, -2}, {0, -2}, {1, -2},  {2, -2},
                               {-2, -1}, {2, -1}, {-2, 0}, {2, 0},  {-2, 1}, {2, 1},   {-2, 2},
                               {-1, 2},  {0, 2},  {1, 2},  {2, 2}};
-// rows 8 m .. 8 m + 7, one per lane of a bead's group: 6 bits each,
-// (dy + 2) | (dz + 2) << 3
+// The centre row (0, 0) is shared by the bead's eight lanes; the other 24
+// go one per lane: rows 8 m + 1 .. 8 m + 8 of the ring order in word m, 6
+// bits each, (dy + 2) | (dz + 2) << 3. Word 0 holds the inner ring.
 constexpr uint64_t ring2_word(int m) {
   uint64_t w = 0;
   for (int k = 0; k < C_LANES; ++k) {
-    const int r = m * C_LANES + k;
+    const int r = m * C_LANES + k + 1;
     if (r < 25) w |= (uint64_t)((RING2[r][0] + 2) | ((RING2[r][1] + 2) << 3)) << (6 * k);
   }
   return w;
 }
-constexpr uint64_t RING2_WORDS[4] = {ring2_word(0), ring2_word(1), ring2_word(2), ring2_word(3)};
+constexpr uint64_t RING2_WORDS[3] = {ring2_word(0), ring2_word(1), ring2_word(2)};
 
 // Pair (and LBS) forces of the bead `me` over its trimmed candidate list,
 // spread over the bead's eight lanes (sub = 0..7) and added to f; on return
@@ -887,7 +888,8 @@ template <int MODE, bool BATCH, int R>
 __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, uint32_t *list, bool active,
                                              uint32_t self, int sub, int lane, float4 me, float &fx, float &fy,
                                              float &fz) {
-  constexpr int W = 2 * R + 1, NROW = W * W, RPL = (NROW + C_LANES - 1) / C_LANES;
+  // NROW: the rows other than the centre one, RPL of them per lane
+  constexpr int W = 2 * R + 1, NROW = W * W - 1, RPL = (NROW + C_LANES - 1) / C_LANES;
   const uint32_t w = wbits(me);
   const float rx = me.x, ry = me.y, rz = me.z;
   bool outside = false;
@@ -907,29 +909,48 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   // so cps and iz cps + iy stay below 2^24
   const int base0 = p.gf.coff + ((cz - R) * cps + (cy - R)) * cps + cx;
 
-  // Reach 2: rows 16.. (the outer ring only) are skipped by a wavefront in
-  // which no bead's cutoff reaches a cell two steps away in y or z.
+  // Reach 2: the outer ring (rows of words 1 and 2) is skipped by a
+  // wavefront in which no bead's cutoff reaches a cell two steps away in y
+  // or z.
   bool any_outer = true;
   if constexpr (R == 2) {
     const float m2 = fminf(gyl, gzl);
     const float near2 = fminf(group_lane(m2, gb4, 0), group_lane(m2, gb4, 2 * R));
     any_outer = __ballot(active && rc2 >= near2) != 0ull;
   }
-  // rows sub, sub + 8, ... (reach 2: in ring order): the byte offsets of
-  // their trimmed slot ranges first, then the start entries of all of them
-  // loaded together, so that a lane waits once for all its rows
+  // The centre row (dy = dz = 0, both gaps 0, so h2 = rc2): the same cells
+  // for the bead's eight lanes, listed first and shared by them.
+  uint32_t oc0, oc1;
+  {
+    const int b = base0 + (R * cps + R) * cps;
+    int c0 = b, c1 = b + 1;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      c0 -= GX.lo[k] <= rc2 ? 1 : 0;
+      c1 += GX.hi[k] <= rc2 ? 1 : 0;
+    }
+    uint32_t a0 = active ? (uint32_t)c0 << 2 : 0u, a1 = active ? (uint32_t)c1 << 2 : 0u;
+    asm("" : "+v"(a0), "+v"(a1));
+    oc0 = a0;
+    oc1 = a1;
+  }
+  // the other rows, sub, sub + 8, ... (reach 2: in ring order): the byte
+  // offsets of their trimmed slot ranges first, then the start entries of
+  // all of them loaded together, so that a lane waits once for all its rows
   uint32_t o0[RPL], o1[RPL];
 #pragma unroll
   for (int m = 0; m < RPL; ++m) {
     o0[m] = 0u;
     o1[m] = 0u;
-    if (R == 2 && m >= 2 && !any_outer) continue;
+    if (R == 2 && m >= 1 && !any_outer) continue;
     // Row (dy, dz) = (iy - R, iz - R): its cells [c0, c1) that can hold a
     // bead within sqrt(rc2), empty when the row's y-z slab is farther (or
     // outside the grid), else the run around cx whose x gaps fit in the rest,
     // rc2 - gap_y^2 - gap_z^2. Branch-free: an empty row selects [0, 0).
     const int r = sub + m * C_LANES;
-    int iy = r % W, iz = r / W;
+    // row r of the W x W rows with the centre (index W * W / 2) left out
+    const int rr = r < W * W / 2 ? r : r + 1;
+    int iy = rr % W, iz = rr / W;
     if constexpr (R == 2) {
       const uint32_t v = (uint32_t)(RING2_WORDS[m] >> (6 * sub)) & 63u;
       iy = (int)(v & 7u);
@@ -954,18 +975,25 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     o1[m] = a1;
   }
   uint32_t lo[RPL], len[RPL], tot = 0;
+  const uint32_t clo = at_byte(q.fcstart, oc0);
+  uint32_t clen = at_byte(q.fcstart, oc1);
 #pragma unroll
   for (int m = 0; m < RPL; ++m) {
     lo[m] = at_byte(q.fcstart, o0[m]);
     len[m] = at_byte(q.fcstart, o1[m]);
   }
+  clen -= clo;
 #pragma unroll
   for (int m = 0; m < RPL; ++m) {
     len[m] -= lo[m];
     tot += len[m];
   }
-  // the LBS sites of LAD beads, row by row
+  // the LBS sites of LAD beads, row by row (the centre row's on lane 0)
   if (!BATCH && MODE != MODE_ICG && p.n_l > 0 && (w & W_LAD)) {
+    if (sub == 0) {
+      const uint32_t llo = at_byte(q.lstart, oc0), lhi = at_byte(q.lstart, oc1);
+      for (uint32_t k = llo; k < lhi; ++k) lbs_force(p, rx, ry, rz, q.lspos[k], fx, fy, fz);
+    }
 #pragma unroll
     for (int m = 0; m < RPL; ++m) {
       const uint32_t llo = at_byte(q.lstart, o0[m]), lhi = at_byte(q.lstart, o1[m]);
@@ -989,11 +1017,14 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   total += dpp_u<DPP_XOR1>(total);
   total += dpp_u<DPP_XOR2>(total);
   total += dpp_u<DPP_MIRROR8>(total);
-  uint32_t off = incl - tot;
+  // the centre row's entries come first
+  total += clen;
+  uint32_t off = clen + incl - tot;
   const bool listed = total <= (uint32_t)C_LIST;
   // the list holds byte offsets (slot x 16) into the ordered positions, so a
   // gather needs no address arithmetic
   if (listed) {
+    for (uint32_t k = sub; k < clen; k += C_LANES) list[k] = (clo + k) << 4;
 #pragma unroll
     for (int m = 0; m < RPL; ++m) {
       uint32_t b = lo[m] << 4;
@@ -1047,7 +1078,9 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
       if (wide) pair_term<MODE>(p, pc, d, fx, fy, fz);
     }
   } else {
-    // crowded neighbourhood: each lane scans its own rows
+    // crowded neighbourhood: each lane scans its own rows, and every eighth
+    // entry of the centre row
+    for (uint32_t k = sub; k < clen; k += C_LANES) pair_term<MODE>(p, pc, q.fspos[clo + k], fx, fy, fz);
 #pragma unroll
     for (int m = 0; m < RPL; ++m)
       for (uint32_t k = 0; k < len[m]; ++k) pair_term<MODE>(p, pc, q.fspos[lo[m] + k], fx, fy, fz);
