@@ -279,6 +279,13 @@ typedef struct ccp_batch_system {
  * lad_placement, graph_steps, ...). NULL on error (ccp_batch_last_error(NULL)). */
 ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *systems, uint32_t n_sys);
 void ccp_batch_destroy(ccp_batch *b);
+/* The layout ccp_batch_create would give these systems, computed on the host
+ * with no device call (the same parameter checks and errors): total beads,
+ * total cells, bucket bytes at the initial cell capacity, and each system's
+ * first cell in the shared cell range (n_sys + 1 entries, the last the
+ * total). Any output may be NULL. Errors: ccp_batch_last_error(NULL). */
+int ccp_batch_layout(const char *common_params, const ccp_batch_system *systems, uint32_t n_sys,
+                     uint32_t *n_beads, uint64_t *n_cells, uint64_t *bucket_bytes, uint64_t *cell_offsets);
 const char *ccp_batch_last_error(const ccp_batch *b);
 /* offsets: n_sys + 1 first bead indices (system s holds beads [off[s], off[s+1])) */
 int ccp_batch_info(const ccp_batch *b, uint32_t *n_sys, uint32_t *n_beads, uint32_t *n_cells, uint32_t *offsets);
@@ -302,10 +309,19 @@ int ccp_batch_reset_samples(ccp_batch *b);
 int ccp_batch_time_steps(ccp_batch *b, uint32_t n, float *ms_per_step);
 
 /* Average device time per launch (ms) of each kernel of the step over
- * n_steps steps, measured with HIP events on the handle's stream:
- * out[0] force stage 1, [1] order (scan + sort + gather) after stage 1,
- * [2] force stage 2, [3] order after stage 2, [4] whole step. */
+ * n_steps steps (at most 100000) that advance the simulation, measured with
+ * HIP events recorded between the launches on the handle's stream (the
+ * kernels run back to back; the host waits once at the end): out[0] force
+ * stage 1, [1] order (scan + sort + gather) after stage 1, [2] force stage
+ * 2, [3] order after stage 2, [4] whole step. Stands in for the timing of
+ * ccp-profile.cu:55-87, which times whole run_simulation calls. */
 int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]);
+
+/* Capture every hipGraph block that ccp_run_steps replays (graph_steps
+ * steps and each power of two below it) ahead of a timed run, so that the
+ * run launches graphs only. No counterpart in ccp (run_simulation,
+ * chrsim.cu:558-593, launches each kernel from the host). */
+int ccp_capture_graphs(ccp_sim *h);
 
 #ifdef __cplusplus
 }

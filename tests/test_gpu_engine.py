@@ -132,7 +132,22 @@ def test_checkpoint_resume_is_bit_exact(tmp_path):
     d = small_sim(seed=5)
     d.load_checkpoint(tmp_path / "ccp.bin")
     assert np.array_equal(d.positions, b.positions)
-    for s in (a, b, c, d):
+    # a handle that already replayed captured graphs under another seed (more
+    # than graph_steps = 64 steps) resumes exactly as a fresh one
+    e = small_sim(seed=4242)
+    e.generate_initial_condition()
+    e.run_steps(200)
+    e.synchronize()
+    e.load_checkpoint(tmp_path / "ic.bin")
+    e.run_steps(150)
+    e.synchronize()
+    f = small_sim(seed=31)
+    f.load_checkpoint(tmp_path / "ic.bin")
+    f.run_steps(150)
+    f.synchronize()
+    assert np.array_equal(e.positions, f.positions)
+    assert np.array_equal(e.positions, b.positions)
+    for s in (a, b, c, d, e, f):
         s.close()
 
 

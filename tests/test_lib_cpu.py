@@ -106,3 +106,56 @@ def test_ensemble_draws():
     for s, i in zip(a, range(4000, 4064)):
         assert s["seed"] == i
         assert 0.1 <= s["volume_fraction"] <= 0.35 and 0.2 <= s["eps_het"] <= 1.0 and 0.2 <= s["eps_lam"] <= 1.5
+
+
+def test_ensemble_layout_fits_the_kernels_index_arithmetic():
+    """The 256-system ensemble (512-16384 beads, seeds 4000 + i) laid out on
+    the host at both kernel shapes: every system's cells are a contiguous
+    range of the shared cell range sized by its own grid (cps^3 with cps =
+    2 ceil(R_n / side), at least 4), the cell indices' byte offsets (index x 4)
+    and the candidate list's slot offsets (slot x 16) fit the 32-bit offsets
+    of the force kernels, the order pass's 256-cell blocks cover the range,
+    and the bucket stays addressable at the largest capacity growth allows
+    (4096 entries per cell)."""
+    import math
+
+    from tfm_amd.ensemble import draw_systems
+
+    systems = draw_systems(256, 512, 16384, 4000)
+    n_total = sum(s["n_beads"] for s in systems)
+    for shape in ({"lanes_per_bead": 3, "cell_reach": 1, "cell_size": 2.5},
+                  {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.375}):
+        lay = tfm_amd.batch_layout(systems, {"timestep": 0.001, **shape})
+        assert lay["beads"] == n_total
+        off = lay["cell_offsets"].astype(np.int64)
+        assert off[0] == 0 and off[-1] == lay["cells"]
+        side = shape["cell_size"]
+        for s, d in enumerate(systems):
+            rco = float(np.float32(1.154701))
+            R_n = float(np.float32(float(f"{rco / 2 + rco / 2 * (d['n_beads'] / d['volume_fraction']) ** (1 / 3):.6f}")))
+            cps = max(2 * math.ceil(np.float32(R_n) / np.float32(side)), 4)
+            assert off[s + 1] - off[s] == cps ** 3, s
+        assert lay["cells"] * 4 < 2 ** 32
+        assert n_total * 16 < 2 ** 32
+        blocks = (lay["cells"] + 255) // 256
+        assert blocks * 256 >= lay["cells"] and blocks < 2 ** 31
+        assert lay["bucket_bytes"] == lay["cells"] * 32 * 16
+        assert lay["cells"] * 4096 * 16 < 2 ** 63
+
+
+def test_half_cell_batches_above_the_tested_size_are_refused():
+    # eight lanes over half-cutoff cells faulted the GPU once on a 1.1M-bead
+    # ensemble with no cause found: the library refuses that shape above
+    # 524288 beads and the batch autotuner leaves it out
+    from tfm_amd import autotune
+    from tfm_amd.ensemble import draw_systems
+
+    systems = draw_systems(256, 512, 16384, 4000)
+    half = {"timestep": 0.001, "lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.25}
+    with pytest.raises(tfm_amd.CcpError, match="limited to 524288 beads per batch"):
+        tfm_amd.batch_layout(systems, half)
+    small = [dict(s, n_beads=2048) for s in systems]
+    assert tfm_amd.batch_layout(small, half)["beads"] == 524288
+    shapes = autotune.candidates(2.5, sum(s["n_beads"] for s in systems))
+    assert {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.25} not in shapes and len(shapes) == 5
+    assert len(autotune.candidates(2.5, 524288)) == 6

@@ -78,6 +78,37 @@ def test_ccp_bending_quirk_is_not_a_gradient():
     assert np.abs(f - g).max() > 1e-3
 
 
+@pytest.mark.parametrize("chains", [[0, 48], [0, 20, 21, 48]])
+def test_ccp_bending_on_mixed_chains_is_the_masked_gradient(chains):
+    # ccp's bending term on bead i (chrsim.cu:65-72) holds the three angle
+    # terms of -grad V_all, V_all = k_b sum_j (1 - cos theta_j) over every
+    # interior angle of the chains, and is applied only where i is type A:
+    # f(k_b) - f(k_b = 0) = 1[i is A] (-grad V_all) (the other force terms
+    # cancel in the difference). A one-bead chain and chain limits included.
+    N = 48
+    fl = pattern(N, 5, mean=4)
+    assert 0 < (fl & po.HET).astype(bool).sum() < N
+    sysm, r = small_system(po.FF_CCP, 21, flags=fl, chain_off=chains)
+    kb = sysm.p.k_b
+    assert kb > 0
+    flat, _ = small_system(po.FF_CCP, 21, flags=fl, chain_off=chains, k_b=0.0)
+    fb = sysm.forces(r, method=1) - flat.forces(r, method=1)
+    g = np.zeros_like(r)
+    h = 1e-6
+    for i in range(N):
+        for a in range(3):
+            rp, rm = r.copy(), r.copy()
+            rp[i, a] += h
+            rm[i, a] -= h
+            g[i, a] = -(sysm.energy(rp)[1] - sysm.energy(rm)[1]) / (2 * h)
+    mask = ((fl & po.HET) != 0)[:, None]
+    scale = np.abs(g).max()
+    assert scale > 0.1
+    assert np.abs(fb - mask * g).max() < 2e-5 * scale + 1e-6
+    # the B beads' part of -grad V_all is what ccp leaves out, and it is not small
+    assert np.abs((~mask) * g).max() > 0.05 * scale
+
+
 @pytest.mark.parametrize("ff", [po.FF_CCP, po.FF_ICG, po.FF_LJ])
 def test_cell_scan_matches_brute_force(ff):
     N = 1500

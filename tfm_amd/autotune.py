@@ -19,10 +19,17 @@ import numpy as np
 from .ccp import Batch, Simulation
 
 
-def candidates(cutoff):
-    """Kernel shapes for a longest pair cutoff `cutoff` (sigma)."""
+# Batches above this many beads refuse eight lanes per bead over half-cutoff
+# cells (reach 2, cells below 1.3 sigma): a larger run of that shape faulted
+# the GPU and its cause is not known (the library gives the same limit).
+HALF_CELL_BATCH_LIMIT = 524288
+
+
+def candidates(cutoff, batch_beads=0):
+    """Kernel shapes for a longest pair cutoff `cutoff` (sigma); for a batch of
+    `batch_beads` beads, without the shapes the library refuses at that size."""
     half = cutoff / 2
-    return [
+    out = [
         {"lanes_per_bead": 3, "cell_reach": 1, "cell_size": cutoff},
         {"lanes_per_bead": 1, "cell_reach": 1, "cell_size": cutoff},
         {"lanes_per_bead": 8, "cell_reach": 1, "cell_size": cutoff},
@@ -30,6 +37,9 @@ def candidates(cutoff):
         {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": half},
         {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": round(half * 1.1, 4)},
     ]
+    if batch_beads > HALF_CELL_BATCH_LIMIT:
+        out = [c for c in out if not (c["lanes_per_bead"] == 8 and c["cell_reach"] == 2 and c["cell_size"] < 1.3)]
+    return out
 
 
 def _key(params, n_flags):
@@ -92,7 +102,7 @@ def tune_batch(systems, common, cutoff=2.5, steps=100, warmup=20, settle=2000, l
     finally:
         ref.close()
     table = []
-    for cand in candidates(cutoff):
+    for cand in candidates(cutoff, sum(int(s["n_beads"]) for s in systems)):
         b = Batch(systems, {**common, **cand})
         try:
             b.set_state(xyz, fl)

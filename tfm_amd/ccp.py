@@ -27,11 +27,12 @@ EXPORTS = (
     "ccp_synchronize", "ccp_get_time", "ccp_set_time", "ccp_get_positions", "ccp_set_positions",
     "ccp_get_types", "ccp_set_types", "ccp_get_flags", "ccp_set_flags", "ccp_get_lbs", "ccp_set_lbs",
     "ccp_get_chains", "ccp_compute_forces", "ccp_cell_list", "ccp_noise", "ccp_count_overlaps",
-    "ccp_contact_sample", "ccp_contact_reset", "ccp_contact_map", "ccp_time_kernels", "ccp_observables",
+    "ccp_contact_sample", "ccp_contact_reset", "ccp_contact_map", "ccp_time_kernels", "ccp_capture_graphs",
+    "ccp_observables",
     "ccp_distance_contact", "ccp_frame_contact_map", "ccp_analysis_create", "ccp_analysis_destroy",
     "ccp_analysis_add_frame", "ccp_analysis_add_trajectory_file", "ccp_analysis_end_simulation",
     "ccp_analysis_finish", "ccp_analysis_last_error", "ccp_series_stats", "ccp_weighted_stats",
-    "ccp_batch_create", "ccp_batch_destroy", "ccp_batch_last_error", "ccp_batch_info", "ccp_batch_system_params",
+    "ccp_batch_create", "ccp_batch_layout", "ccp_batch_destroy", "ccp_batch_last_error", "ccp_batch_info", "ccp_batch_system_params",
     "ccp_batch_generate_initial_conditions", "ccp_batch_run_steps", "ccp_batch_synchronize", "ccp_batch_get_state",
     "ccp_batch_set_state", "ccp_batch_get_step", "ccp_batch_set_step", "ccp_batch_contact_maps",
     "ccp_batch_radial_density", "ccp_batch_reset_samples", "ccp_batch_time_steps", "ccp_sequence_pattern",
@@ -113,6 +114,7 @@ def lib():
         L.ccp_count_overlaps.argtypes = [vp, ctypes.c_float, P(ctypes.c_uint64)]
         L.ccp_contact_map.argtypes = [vp, u32p, sz, u32p]
         L.ccp_time_kernels.argtypes = [vp, ctypes.c_uint32, f32p]
+        L.ccp_capture_graphs.argtypes = [vp]
         f64p = P(ctypes.c_double)
         L.ccp_observables.argtypes = [vp, f64p, f64p, sz]
         L.ccp_distance_contact.argtypes = [vp, ctypes.c_uint32, f32p, f32p, sz]
@@ -129,6 +131,8 @@ def lib():
         L.ccp_weighted_stats.argtypes = [f64p, f64p, sz, f64p]
         L.ccp_batch_create.argtypes = [ctypes.c_char_p, P(BatchSystem), ctypes.c_uint32]
         L.ccp_batch_create.restype = vp
+        L.ccp_batch_layout.argtypes = [ctypes.c_char_p, P(BatchSystem), ctypes.c_uint32, u32p,
+                                       P(ctypes.c_uint64), P(ctypes.c_uint64), P(ctypes.c_uint64)]
         L.ccp_batch_destroy.argtypes = [vp]
         L.ccp_batch_last_error.argtypes = [vp]
         L.ccp_batch_last_error.restype = ctypes.c_char_p
@@ -348,6 +352,10 @@ class Simulation:
         self._check(lib().ccp_time_kernels(self.h, n_steps, _ptr(out, ctypes.c_float)))
         return out
 
+    def capture_graphs(self):
+        """Capture every step block run_steps replays, ahead of a timed run."""
+        self._check(lib().ccp_capture_graphs(self.h))
+
     def observables(self, n_bins=64):
         """ccp's frame observables: ((dcm, rg2, nop, ncf), rcd[3, n_bins])."""
         sc = np.zeros(4)
@@ -431,6 +439,36 @@ def weighted_stats(avg, sem):
     return out
 
 
+def _batch_systems(systems):
+    arr = (BatchSystem * len(systems))()
+    for a, d in zip(arr, systems):
+        a.n_beads = int(d["n_beads"])
+        a.n_chains = int(d.get("n_chains", 1))
+        a.volume_fraction = float(d["volume_fraction"])
+        a.eps_het = float(d.get("eps_het", 0.5))
+        a.eps_lam = float(d.get("eps_lam", 1.0))
+        a.het_fraction = float(d.get("het_fraction", 0.3))
+        a.lad_fraction = float(d.get("lad_fraction", 0.1))
+        a.seed = int(d["seed"])
+        a.replica = int(d.get("replica", 0))
+    return arr
+
+
+def batch_layout(systems, common=None):
+    """The layout a Batch of these systems would have, computed on the host
+    (no GPU): dict with beads, cells, bucket_bytes and cell_offsets (first
+    cell of each system, then the total). Raises CcpError as Batch would."""
+    arr = _batch_systems(systems)
+    nb = ctypes.c_uint32()
+    nc, bb = ctypes.c_uint64(), ctypes.c_uint64()
+    off = np.zeros(len(systems) + 1, np.uint64)
+    rc = lib().ccp_batch_layout(params_text(common or {}).encode(), arr, len(systems), ctypes.byref(nb),
+                                ctypes.byref(nc), ctypes.byref(bb), _ptr(off, ctypes.c_uint64))
+    if rc != 0:
+        raise CcpError(rc, lib().ccp_batch_last_error(None).decode())
+    return {"beads": nb.value, "cells": nc.value, "bucket_bytes": bb.value, "cell_offsets": off}
+
+
 class Batch:
     """Independent systems of the WCA/LJ set stepped together on one GPU.
 
@@ -439,17 +477,7 @@ class Batch:
 
     def __init__(self, systems, common=None):
         self.h = None
-        arr = (BatchSystem * len(systems))()
-        for a, d in zip(arr, systems):
-            a.n_beads = int(d["n_beads"])
-            a.n_chains = int(d.get("n_chains", 1))
-            a.volume_fraction = float(d["volume_fraction"])
-            a.eps_het = float(d.get("eps_het", 0.5))
-            a.eps_lam = float(d.get("eps_lam", 1.0))
-            a.het_fraction = float(d.get("het_fraction", 0.3))
-            a.lad_fraction = float(d.get("lad_fraction", 0.1))
-            a.seed = int(d["seed"])
-            a.replica = int(d.get("replica", 0))
+        arr = _batch_systems(systems)
         h = lib().ccp_batch_create(params_text(common or {}).encode(), arr, len(systems))
         if not h:
             raise CcpError(1, lib().ccp_batch_last_error(None).decode())

@@ -414,79 +414,99 @@ std::string system_text(const std::string &common, const ccp_batch_system &s, ui
   return common + "\n" + buf;
 }
 
+// The half-cutoff cells at eight lanes per bead (reach 2, cells below
+// 1.3 sigma) have run clean in batches of up to this many beads (256 chains of
+// 2048). A command that ran a 1.1M-bead ensemble of this shape for 1e4 steps,
+// after initial conditions generated on 16 host threads, faulted the GPU; its
+// cause is not known, so larger batches of this shape are refused.
+constexpr uint64_t HALF_CELL_BATCH_LIMIT = 524288;
+
+// Everything of a batch that needs no device: each system's parameter text,
+// constants, geometry and place in the shared bead and cell ranges.
+void plan_batch(ccp_batch &bt, const char *common_params, const ccp_batch_system *systems, uint32_t n_sys) {
+  ccp_batch *b = &bt;
+  if (!systems || n_sys == 0) throw Error(CCP_ERR_ARG, "no systems");
+  const std::string common = common_params ? common_params : "";
+  ParMap cpar(common.c_str());
+  b->bins = cpar.get<uint32_t>("contact_bins", 128u);
+  b->rd_bins = cpar.get<uint32_t>("radial_bins", 32u);
+  if (!(1 <= b->bins && b->bins <= 4096)) throw Error(CCP_ERR_PARAM, "contact_bins out of range");
+  if (!(1 <= b->rd_bins && b->rd_bins <= 1024)) throw Error(CCP_ERR_PARAM, "radial_bins out of range");
+  b->n_sys = n_sys;
+  b->slot.push_back(0);
+  int coff = 0;
+  uint64_t total = 0;
+  for (uint32_t s = 0; s < n_sys; ++s) {
+    const ccp_batch_system &in = systems[s];
+    if (!(in.n_beads >= 4 && in.volume_fraction > 0.0f && in.volume_fraction < 0.74f))
+      throw Error(CCP_ERR_PARAM, "batch system out of range");
+    b->text.push_back(system_text(common, in, b->bins));
+    // the system's own configuration gives its geometry and constants
+    ccp_sim one;
+    one.log_fn = quiet;
+    ParMap par(b->text.back().c_str());
+    one.configure(par);
+    one.fill_kparams();
+    if (one.split) throw Error(CCP_ERR_PARAM, "fine_cell_size is not supported in batches");
+    SysParams S{};
+    S.R_n = one.R_n; S.R_o = one.R_o; S.R_b = one.R_b;
+    S.noc = one.noc; S.boc = one.boc; S.nod = one.nod; S.bod = one.bod;
+    S.eps_het = one.eps_het; S.eps_lam = one.eps_lam;
+    S.seed_lo = one.kp.seed_lo;
+    S.seed_hi = one.kp.seed_hi;
+    S.slot = (uint32_t)total;
+    S.n = one.N;
+    S.g = one.gf.g;
+    S.g.coff = coff;
+    if (s == 0) {
+      b->kp = one.kp;
+      b->contact_rc = one.contact_rc;
+      b->cinterval = one.cinterval;
+      b->graph_steps = one.graph_steps;
+      b->spf = one.spf;
+      b->integrate_pass = one.integrate_pass;
+    } else if (one.kp.lanes != b->kp.lanes || one.kp.reach != b->kp.reach || one.gf.g.csl != b->kp.gf.csl ||
+               one.gf.g.cap != b->kp.gf.cap || one.kp.dt != b->kp.dt || one.kp.sd != b->kp.sd) {
+      throw Error(CCP_ERR_PARAM, "batch systems must share the kernel shape, timestep and temperature");
+    }
+    one.hfl.assign(one.N, 0);
+    for (uint32_t i = 0; i < one.N; ++i) b->hbond.push_back(one.wbits(i) & (W_B0 | W_B1 | W_B2 | W_B3));
+    b->sys.push_back(S);
+    b->nchain.push_back((uint32_t)one.chain_off.size() - 1);
+    // cell indices stay below 2^30 so that their byte offsets (index x 4)
+    // fit the 32-bit offsets of the force kernels' row loads
+    if ((uint64_t)coff + (uint64_t)one.gf.g.n_c >= (1ull << 30)) throw Error(CCP_ERR_PARAM, "batch too large");
+    coff += one.gf.g.n_c;
+    total += one.N;
+    if (total >= (1ull << ID_BITS)) throw Error(CCP_ERR_PARAM, "batch too large");
+    b->slot.push_back((uint32_t)total);
+  }
+  if (b->kp.lanes == 8 && b->kp.reach == 2 && b->kp.gf.csl < 1.3f && total > HALF_CELL_BATCH_LIMIT)
+    throw Error(CCP_ERR_PARAM, "lanes_per_bead 8 with cell_reach 2 and cells below 1.3 sigma is limited to "
+                               "524288 beads per batch: a larger run of this shape faulted the GPU and the cause "
+                               "is not yet found");
+  b->N = (uint32_t)total;
+  b->kp.N = (int)b->N;
+  b->kp.n_l = 0;
+  b->kp.split = 0;
+  b->kp.gh.n_c = 0;
+  b->grid.g = b->kp.gf;
+  b->grid.g.n_c = coff;
+  b->grid.g.coff = 0;
+  b->grid.g.nblk = (coff + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  b->hr.assign(3 * (size_t)b->N, 0.0f);
+  b->hfl.assign(b->N, 0);
+  b->rd_sum.assign((size_t)n_sys * b->rd_bins, 0.0);
+}
+
 } // namespace
 
 extern "C" {
 
 ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *systems, uint32_t n_sys) {
   try {
-    if (!systems || n_sys == 0) throw Error(CCP_ERR_ARG, "no systems");
     std::unique_ptr<ccp_batch> b(new ccp_batch());
-    const std::string common = common_params ? common_params : "";
-    ParMap cpar(common.c_str());
-    b->bins = cpar.get<uint32_t>("contact_bins", 128u);
-    b->rd_bins = cpar.get<uint32_t>("radial_bins", 32u);
-    if (!(1 <= b->bins && b->bins <= 4096)) throw Error(CCP_ERR_PARAM, "contact_bins out of range");
-    if (!(1 <= b->rd_bins && b->rd_bins <= 1024)) throw Error(CCP_ERR_PARAM, "radial_bins out of range");
-    b->n_sys = n_sys;
-    b->slot.push_back(0);
-    int coff = 0;
-    uint64_t total = 0;
-    for (uint32_t s = 0; s < n_sys; ++s) {
-      const ccp_batch_system &in = systems[s];
-      if (!(in.n_beads >= 4 && in.volume_fraction > 0.0f && in.volume_fraction < 0.74f))
-        throw Error(CCP_ERR_PARAM, "batch system out of range");
-      b->text.push_back(system_text(common, in, b->bins));
-      // the system's own configuration gives its geometry and constants
-      ccp_sim one;
-      one.log_fn = quiet;
-      ParMap par(b->text.back().c_str());
-      one.configure(par);
-      one.fill_kparams();
-      if (one.split) throw Error(CCP_ERR_PARAM, "fine_cell_size is not supported in batches");
-      SysParams S{};
-      S.R_n = one.R_n; S.R_o = one.R_o; S.R_b = one.R_b;
-      S.noc = one.noc; S.boc = one.boc; S.nod = one.nod; S.bod = one.bod;
-      S.eps_het = one.eps_het; S.eps_lam = one.eps_lam;
-      S.seed_lo = one.kp.seed_lo;
-      S.seed_hi = one.kp.seed_hi;
-      S.slot = (uint32_t)total;
-      S.n = one.N;
-      S.g = one.gf.g;
-      S.g.coff = coff;
-      if (s == 0) {
-        b->kp = one.kp;
-        b->contact_rc = one.contact_rc;
-        b->cinterval = one.cinterval;
-        b->graph_steps = one.graph_steps;
-        b->spf = one.spf;
-        b->integrate_pass = one.integrate_pass;
-      } else if (one.kp.lanes != b->kp.lanes || one.kp.reach != b->kp.reach || one.gf.g.csl != b->kp.gf.csl ||
-                 one.gf.g.cap != b->kp.gf.cap || one.kp.dt != b->kp.dt || one.kp.sd != b->kp.sd) {
-        throw Error(CCP_ERR_PARAM, "batch systems must share the kernel shape, timestep and temperature");
-      }
-      one.hfl.assign(one.N, 0);
-      for (uint32_t i = 0; i < one.N; ++i) b->hbond.push_back(one.wbits(i) & (W_B0 | W_B1 | W_B2 | W_B3));
-      b->sys.push_back(S);
-      b->nchain.push_back((uint32_t)one.chain_off.size() - 1);
-      coff += one.gf.g.n_c;
-      total += one.N;
-      if (total >= (1ull << ID_BITS) || (uint64_t)coff >= (1ull << 31))
-        throw Error(CCP_ERR_PARAM, "batch too large");
-      b->slot.push_back((uint32_t)total);
-    }
-    b->N = (uint32_t)total;
-    b->kp.N = (int)b->N;
-    b->kp.n_l = 0;
-    b->kp.split = 0;
-    b->kp.gh.n_c = 0;
-    b->grid.g = b->kp.gf;
-    b->grid.g.n_c = coff;
-    b->grid.g.coff = 0;
-    b->grid.g.nblk = (coff + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    b->hr.assign(3 * (size_t)b->N, 0.0f);
-    b->hfl.assign(b->N, 0);
-    b->rd_sum.assign((size_t)n_sys * b->rd_bins, 0.0);
+    plan_batch(*b, common_params, systems, n_sys);
     // device
     hip_check(hipGetDevice(&b->dev));
     hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -515,6 +535,28 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
   } catch (const std::exception &e) {
     g_batch_create_error = e.what();
     return nullptr;
+  }
+}
+
+int ccp_batch_layout(const char *common_params, const ccp_batch_system *systems, uint32_t n_sys,
+                     uint32_t *n_beads, uint64_t *n_cells, uint64_t *bucket_bytes, uint64_t *cell_offsets) {
+  try {
+    ccp_batch b;
+    plan_batch(b, common_params, systems, n_sys);
+    if (n_beads) *n_beads = b.N;
+    if (n_cells) *n_cells = (uint64_t)b.grid.g.n_c;
+    if (bucket_bytes) *bucket_bytes = (uint64_t)b.grid.g.n_c * (uint64_t)b.grid.g.cap * sizeof(float4);
+    if (cell_offsets) {
+      for (uint32_t s = 0; s < n_sys; ++s) cell_offsets[s] = (uint64_t)b.sys[s].g.coff;
+      cell_offsets[n_sys] = (uint64_t)b.grid.g.n_c;
+    }
+    return CCP_OK;
+  } catch (const Error &e) {
+    g_batch_create_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_batch_create_error = e.what();
+    return CCP_ERR_ARG;
   }
 }
 

@@ -447,33 +447,49 @@ void ccp_sim::launch_stage(int mode, int stage, hipEvent_t mid) {
   gx.valid = false;
 }
 
-// n steps without contact sampling: whole graphs of graph_steps, then eager.
-void ccp_sim::run_chunk(uint64_t n, int mode) {
-  if (n == 0) return;
-  const uint64_t G = graph_steps;
-  if (n >= G) {
-    hipGraphExec_t &ge = graphs[mode][gf.clean][gh.clean];
+// The captured block of `len` steps for the current buffer parity (captured
+// on first use).
+hipGraphExec_t ccp_sim::step_graph(uint32_t len, int mode) {
+  hipGraphExec_t &ge = graphs[graph_key(mode, gf.clean, gh.clean, len)];
+  if (!ge) {
     const int cf = gf.clean, ch = gh.clean;
     const bool vh = gh.valid;
-    if (!ge) {
-      hipGraph_t g;
-      hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      for (uint64_t s = 0; s < G; ++s) launch_step(mode);
-      hip_check(hipStreamEndCapture(stream, &g));
-      hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-      hip_check(hipGraphDestroy(g));
-    }
-    while (n >= G) {
-      hip_check(hipGraphLaunch(ge, stream));
-      n -= G;
-    }
+    hipGraph_t g;
+    hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t s = 0; s < len; ++s) launch_step(mode);
+    hip_check(hipStreamEndCapture(stream, &g));
+    hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    hip_check(hipGraphDestroy(g));
     // each step flips the clean buffers twice
     gf.clean = cf;
     gh.clean = ch;
     gh.valid = mode != MODE_ICG ? vh : false;
   }
-  for (uint64_t s = 0; s < n; ++s) launch_step(mode);
+  return ge;
+}
+
+// n steps without contact sampling: whole graphs of graph_steps, then the
+// rest as graphs of the powers of two in its binary expansion.
+void ccp_sim::run_chunk(uint64_t n, int mode) {
+  if (n == 0) return;
+  const uint64_t G = graph_steps;
+  const int cf = gf.clean, ch = gh.clean;
+  const bool vh = gh.valid;
+  if (n >= G) {
+    hipGraphExec_t ge = step_graph((uint32_t)G, mode);
+    for (; n >= G; n -= G) hip_check(hipGraphLaunch(ge, stream));
+  }
+  for (uint32_t len = 1u << 12; len > 0; len >>= 1)
+    if (n & len) hip_check(hipGraphLaunch(step_graph(len, mode), stream));
+  gf.clean = cf;
+  gh.clean = ch;
+  gh.valid = mode != MODE_ICG ? vh : false;
   hip_check(hipGetLastError());
+}
+
+void ccp_sim::capture_graphs(int mode) {
+  for (uint32_t len = 1; len < graph_steps; len <<= 1) step_graph(len, mode);
+  step_graph(graph_steps, mode);
 }
 
 void ccp_sim::run_steps(uint64_t n, int mode) {
@@ -911,6 +927,11 @@ int ccp_load_checkpoint(ccp_sim *h, const char *path) {
         }
       }
     }
+    // Captured graphs hold the kernel parameters (the noise key among them)
+    // by value, and znoise may hold normals of the old key for this step:
+    // both are rebuilt from the loaded state.
+    h->drop_graphs();
+    h->noise_step = ~0ull;
     h->dstep_dirty = true;
     h->upload_positions();
     h->upload_lbs();
@@ -1314,42 +1335,52 @@ int ccp_frame_contact_map(ccp_sim *h, float *cm, size_t lcm) {
 
 int ccp_time_kernels(ccp_sim *h, uint32_t n_steps, float out[5]) {
   return guarded(h, [&] {
-    need(out && n_steps > 0, "bad timing request");
+    need(out && n_steps > 0 && n_steps <= 100000, "bad timing request");
     const int mode = h->ff == CCP_FF_LJ ? MODE_LJ : MODE_CCP;
     h->ensure_grids();
     h->prime_terms(mode);
-    hipEvent_t ev[5];
+    // Events before the force launch, between it and the order launch, and
+    // after the order launch of every stage, all recorded in the stream with
+    // no wait in between, so the kernels run back to back as in a run; the
+    // host reads them once at the end.
+    std::vector<hipEvent_t> ev(6 * (size_t)n_steps);
     for (auto &e : ev) hip_check(hipEventCreate(&e));
-    double acc[4] = {0, 0, 0, 0};
-    for (uint32_t s = 0; s < n_steps; ++s) {
+    size_t k = 0;
+    for (uint32_t s = 0; s < n_steps; ++s)
       for (int stage = 1; stage <= 2; ++stage) {
-        // force launch | order launch, bracketed by events
-        hip_check(hipEventRecord(ev[0], h->stream));
-        h->launch_stage(mode, stage, ev[1]);
-        hip_check(hipEventRecord(ev[2], h->stream));
-        hip_check(hipEventSynchronize(ev[2]));
+        hip_check(hipEventRecord(ev[k], h->stream));
+        h->launch_stage(mode, stage, ev[k + 1]);
+        hip_check(hipEventRecord(ev[k + 2], h->stream));
+        k += 3;
+      }
+    hip_check(hipEventSynchronize(ev.back()));
+    double acc[4] = {0, 0, 0, 0};
+    k = 0;
+    for (uint32_t s = 0; s < n_steps; ++s)
+      for (int stage = 1; stage <= 2; ++stage) {
         float a, b;
-        hip_check(hipEventElapsedTime(&a, ev[0], ev[1]));
-        hip_check(hipEventElapsedTime(&b, ev[1], ev[2]));
+        hip_check(hipEventElapsedTime(&a, ev[k], ev[k + 1]));
+        hip_check(hipEventElapsedTime(&b, ev[k + 1], ev[k + 2]));
         acc[(stage - 1) * 2] += a;
         acc[(stage - 1) * 2 + 1] += b;
+        k += 3;
       }
-    }
-    h->step += n_steps;
-    h->noise_step = h->step;
-    // whole steps back to back (no events between kernels)
-    hip_check(hipEventRecord(ev[3], h->stream));
-    for (uint32_t s = 0; s < n_steps; ++s) h->launch_step(mode);
-    hip_check(hipEventRecord(ev[4], h->stream));
-    hip_check(hipEventSynchronize(ev[4]));
     float whole;
-    hip_check(hipEventElapsedTime(&whole, ev[3], ev[4]));
+    hip_check(hipEventElapsedTime(&whole, ev.front(), ev.back()));
     h->step += n_steps;
     h->noise_step = h->step;
-    for (int k = 0; k < 4; ++k) out[k] = (float)(acc[k] / n_steps);
+    for (int j = 0; j < 4; ++j) out[j] = (float)(acc[j] / n_steps);
     out[4] = whole / n_steps;
     for (auto &e : ev) (void)hipEventDestroy(e);
     h->check_flags();
+  });
+}
+
+int ccp_capture_graphs(ccp_sim *h) {
+  return guarded(h, [&] {
+    const int mode = h->ff == CCP_FF_LJ ? MODE_LJ : MODE_CCP;
+    h->ensure_grids();
+    h->capture_graphs(mode);
   });
 }
 

@@ -11,6 +11,7 @@
 #include <ctime>
 #include <fstream>
 #include <iostream>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -167,7 +168,13 @@ struct ccp_sim {
   GridGeom lg{}; // LBS sites on the fine geometry (uses gf's binning buffers)
   HostGrid gh; // coarse grid of het beads
   HostGrid gx; // coarse grid of every bead in ccp's geometry (cell-list export)
-  hipGraphExec_t graphs[3][2][2] = {};
+  // Captured step blocks by (mode, clean buffer of each grid, length): the
+  // length is graph_steps or a power of two below it, so a run of any length
+  // replays graphs only.
+  std::map<uint32_t, hipGraphExec_t> graphs;
+  static uint32_t graph_key(int mode, int cf, int ch, uint32_t len) {
+    return (uint32_t)mode | (uint32_t)cf << 2 | (uint32_t)ch << 3 | len << 4;
+  }
 
   // misc
   std::string err;
@@ -175,13 +182,8 @@ struct ccp_sim {
   void *log_user = nullptr;
 
   void drop_graphs() {
-    for (auto &gm : graphs)
-      for (auto &gr : gm)
-        for (auto &g : gr)
-          if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-          }
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
   }
 
   ~ccp_sim() {
@@ -268,6 +270,9 @@ struct ccp_sim {
   }
   void run_steps(uint64_t n, int mode);
   void run_chunk(uint64_t n, int mode);
+  hipGraphExec_t step_graph(uint32_t len, int mode);
+  // every block length run_chunk can replay, captured ahead of a timed run
+  void capture_graphs(int mode);
   // -------------------------------------------------------------- initial condition
   void set_lbs_positions(HostRng &g);
   void set_particle_types(HostRng &g);

@@ -52,7 +52,8 @@ def run(argv=None):
     ap.add_argument("--interval", type=int, default=1000)
     ap.add_argument("--graph-steps", type=int, default=100)
     ap.add_argument("--seed0", type=int, default=4000)
-    ap.add_argument("--no-tune", action="store_true", help="use the 8-lane, reach-2, 1.25 sigma shape")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the 8-lane, reach-2, 1.25 sigma shape (refused above 524288 beads)")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
