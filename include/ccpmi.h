@@ -163,6 +163,15 @@ int ccp_compute_forces(ccp_sim *h, int ff, float *f_xyz, size_t n);
  * bead indices (n = N), beg/end per cell (n_c entries, empty cells 0/0). */
 int ccp_cell_list(ccp_sim *h, uint32_t *sci, uint32_t *soi, uint32_t *beg, uint32_t *end, size_t n, size_t n_c);
 
+/* The cell list the step itself uses for the current positions (the grid
+ * of cell_size, or fine_cell_size when split, whose lookup multiplies by
+ * 1/cell_size): cstart (fine_cps^3 + 1 entries) the first slot of every
+ * cell, soi (N) the bead index of every slot. The last Heun stage bins and
+ * orders it; after ccp_set_positions it is built afresh. Stands in for the
+ * sorted arrays of sugrid::generate_arrays (sugrid.cu:125-133) as the step
+ * sees them. */
+int ccp_step_grid(ccp_sim *h, uint32_t *cstart, size_t n_c1, uint32_t *soi, size_t n);
+
 /* The noise of step `step` for every bead, in units of one standard
  * deviation (begin_iter, chrsim.cu:348-356), and the redraw count. */
 int ccp_noise(ccp_sim *h, uint64_t step, float *z_xyz, uint32_t *attempts, size_t n);

@@ -135,6 +135,19 @@ template <class R> inline R lj_fac(R eps, R sig2, R d2) {
   return (R)24 * eps * s6 * ((R)2 * s6 - (R)1) / d2;
 }
 
+// ccp's Wang-Frenkel factor e (18 d^4 - 96 d^2 + 96) / d^8 with its
+// arithmetic (calc_ppf<DST>, chrsim.cu:184; calc_wpf, :87): d2 = d * d in R,
+// the polynomial and the division in double (ccp's double literals), d^8 the
+// R product ((d2 d2) d2) d2, the factor rounded to R. In FP64 every step is
+// double, so this is the plain formula.
+template <class R> inline R ccp_wf(R e, R d) {
+  const R d2 = d * d;
+  const double x = (double)d2;
+  const double poly = 18.0 * x * x - 96.0 * x + 96.0;
+  const R d8 = d2 * d2 * d2 * d2;
+  return (R)((double)e * poly / (double)d8);
+}
+
 // ---------------------------------------------------------------- forces
 // Bonded forces on bead i: harmonic (or FENE) bonds and cosine bending,
 // following calc_bf (chrsim.cu:13-76) with per-chain limits.
@@ -226,8 +239,7 @@ template <class R> V3<R> wall_force(const Sys<R> &S, V3<R> ri, uint8_t fli) {
   if (p->ff != ORC_FF_LJ) { // calc_wpf, chrsim.cu:79-88 (Wang-Frenkel, eps = 1)
     R dwp = len(vwp);
     if (dwp > (R)p->rco) return {0, 0, 0};
-    R d2 = dwp * dwp;
-    return (((R)18 * d2 * d2 - (R)96 * d2 + (R)96) / (d2 * d2 * d2 * d2)) * vwp;
+    return ccp_wf((R)1, dwp) * vwp;
   }
   R d2 = dot(vwp, vwp);
   R sig2 = (R)(p->sigma * p->sigma), rm2 = (R)(std::pow(2.0, 1.0 / 3.0) * p->sigma * p->sigma);
@@ -245,8 +257,7 @@ template <class R> inline V3<R> pair_force(const Sys<R> &S, V3<R> vpp, uint8_t f
     R dpp = len(vpp);
     if (dpp > (R)p->aco) return {0, 0, 0};
     if (!hhi && dpp > (R)p->rco) return {0, 0, 0};
-    R d2 = dpp * dpp;
-    return ((R)p->e_p * ((R)18 * d2 * d2 - (R)96 * d2 + (R)96) / (d2 * d2 * d2 * d2)) * vpp;
+    return ccp_wf((R)p->e_p, dpp) * vpp;
   }
   if (p->ff == ORC_FF_ICG) {
     R dpp = len(vpp);
@@ -267,9 +278,11 @@ template <class R> inline V3<R> lbs_force(const Sys<R> &S, V3<R> vlp) {
   R dlp = len(vlp);
   R lco = (R)p->lco;
   if (dlp > lco) return {0, 0, 0};
+  // ccp's arithmetic: nd and its powers in R, the rest in double
   R nd = dlp / lco;
   R d2 = nd * nd;
-  return ((R)p->e_l * ((R)8 / ((R)3 * lco * lco)) * (d2 * d2 * d2 - (R)1)) * vlp;
+  const double lfac = (double)(R)p->e_l * (8.0 / (3.0 * (double)lco * (double)lco));
+  return (R)(lfac * ((double)(d2 * d2 * d2) - 1.0)) * vlp;
 }
 
 // Sorted uniform grid over positions in R (sugrid.cu:15-77 semantics).

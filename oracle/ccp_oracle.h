@@ -6,7 +6,10 @@
  * links, calls or falls back to it.
  *
  * It is a scalar FP64 (and FP32) restatement of the algorithm in the
- * reference program `ccp` (Marco-Mendivil-Carboni/TFM, Program/src):
+ * reference program `ccp` (Marco-Mendivil-Carboni/TFM, Program/src). Its
+ * FP32 mode keeps ccp's mixed widths: the Wang-Frenkel and binding factors
+ * are evaluated in FP64 where ccp's double literals promote them
+ * (chrsim.cu:87,184,211):
  *   - bonded, bending, wall, pair and lamina-binding forces: chrsim.cu:13-317
  *   - stochastic Heun step:                                   chrsim.cu:336-416
  *   - sorted uniform grid (cell list):                        sugrid.cu:15-133
@@ -23,9 +26,9 @@
  *                   forms), cutoff zeros, brute-force O(N^2) pair sums
  *   integrator      free diffusion variance 2 kT dt, harmonic dimer
  *   contacts        brute-force O(N^2) counts
- * The ccp bending term for mixed A/B chains is not a gradient; it is pinned
- * for all-A chains by finite differences and otherwise by transcription
- * ("parity unpinned" for the mixed-chain quirk beyond that).
+ * The ccp bending term for mixed A/B chains is not a gradient: it is pinned
+ * as the gradient of the all-angle bending energy masked to the A beads,
+ * 1[i is A] (-grad V_all), by finite differences.
  */
 #ifndef CCP_ORACLE_H
 #define CCP_ORACLE_H

@@ -148,3 +148,84 @@ def test_ensemble_runner_small():
     assert out["contact_samples"] == 4
     assert out["contacts_per_system_mean"] > 0 and out["bead_steps_per_s"] > 0
     assert len(out["radial_density_mean"]) == 32 and min(out["radial_density_mean"]) >= 0
+
+
+ENSEMBLE_COMMON = dict(timestep=0.001, contact_interval=500, contact_cutoff=1.5, contact_bins=128, radial_bins=32,
+                       graph_steps=100)
+
+
+def test_ensemble_full_size():
+    # the ensemble workload at full size: 256 single-chain systems of
+    # 512-16384 beads (log-uniform, seeds 4000 + i, 1.1M beads), vf, eps_het
+    # and eps_lamina drawn per system, in the shape the batch autotuner picks
+    # at this size (3 lanes per bead over 2.5 sigma cells), 2000 graph-replayed
+    # steps sampled every 500
+    from tfm_amd.ensemble import draw_systems
+
+    systems = draw_systems(256, 512, 16384, 4000)
+    step(f"ensemble full size: {len(systems)} systems, {sum(s['n_beads'] for s in systems)} beads")
+    b = tfm_amd.Batch(systems, dict(ENSEMBLE_COMMON, lanes_per_bead=3, cell_reach=1, cell_size=2.5))
+    b.generate_initial_conditions()
+    xyz0, fl0 = b.state()
+    lengths = np.array([s["n_beads"] for s in systems])
+    check = [int(np.argmin(lengths)), int(np.argmax(lengths)), 17, 200]
+    geo = {}
+    for s in check:
+        sim = tfm_amd.Simulation(b.system_params(s))
+        geo[s] = (float(sim.info.R_n), float(sim.info.R_o))
+        sim.close()
+    want_rd = {s: np.zeros(32) for s in check}
+    for k in range(4):
+        b.run_steps(500)
+        xyz, fl = b.state()
+        for s in check:
+            o0, o1 = b.offsets[s], b.offsets[s + 1]
+            R_n, R_o = geo[s]
+            nod = float(np.float32(np.sqrt(R_n * R_n - R_o * R_o)))
+            want_rd[s] += po.observables(xyz[o0:o1].astype(np.float64), fl[o0:o1], R_n, nod=nod, n_b=32)[1][2]
+        step(f"  {500 * (k + 1)} steps")
+    b.synchronize()
+    maps, ns = b.contact_maps()
+    rd = b.radial_density()
+    assert ns == 4 and b.step == 2000
+    assert np.isfinite(xyz).all()
+    assert maps.shape == (256, 128, 128)
+    assert all(np.array_equal(m, m.T) for m in maps)
+    assert (maps.reshape(256, -1).sum(axis=1) > 0).all()
+    # each system's mean radial density integrates to its beads' volume
+    # (less the small part of wall beads' spheres outside the nucleus)
+    vb = 4.0 / 3.0 * np.pi * (float(np.float32(1.154701)) / 2) ** 3
+    for s in range(256):
+        sim_R = float(np.float32(float(f"{0.5773505 + 0.5773505 * (lengths[s] / systems[s]['volume_fraction']) ** (1 / 3):.6f}")))
+        shell = 4.0 / 3.0 * np.pi * sim_R ** 3 / 32
+        tot = rd[s].sum() * shell / (lengths[s] * vb)
+        assert 0.95 < tot <= 1.0 + 1e-9, (s, tot)
+    for s in check:
+        np.testing.assert_allclose(rd[s], want_rd[s] / 4, rtol=1e-9, atol=1e-12)
+    # four systems (the shortest, the longest, two others) alone, bit for bit
+    for s in check:
+        sim = single_from(b, s, xyz0, fl0)
+        sim.run_steps(2000)
+        sim.synchronize()
+        o0, o1 = b.offsets[s], b.offsets[s + 1]
+        assert np.array_equal(sim.positions, xyz[o0:o1]), s
+        m, n1 = sim.contact_map()
+        k = m.shape[0]
+        assert n1 == 4 and np.array_equal(m, maps[s][:k, :k]), s
+        sim.close()
+    b.close()
+
+
+@pytest.mark.skip(reason="eight lanes over 1.25 sigma cells on the 1.1M-bead ensemble faulted the GPU once (with "
+                         "initial conditions generated on 16 host threads) and the cause is not found; the library "
+                         "refuses this shape above 524288 beads until it is")
+def test_ensemble_full_size_half_cells():
+    from tfm_amd.ensemble import draw_systems
+
+    systems = draw_systems(256, 512, 16384, 4000)
+    b = tfm_amd.Batch(systems, dict(ENSEMBLE_COMMON, lanes_per_bead=8, cell_reach=2, cell_size=1.25))
+    b.generate_initial_conditions()
+    b.run_steps(10000)
+    xyz, _ = b.state()
+    assert np.isfinite(xyz).all()
+    b.close()

@@ -1,25 +1,40 @@
 """Long-run observables of the GPU engine agree with the FP64 oracle within
-statistical error: the same 512-bead system (ccp's force field, LBS sites,
-dt = 1/2048) run for 160000 steps on the GPU and in the oracle with another
-noise key; time averages of Rg^2, the 8-shell radial density and the contact
-probability P(s) at a few separations are compared with their blocking
-errors (ccp's statistics, oracle/sostat.py)."""
+statistical error, for ccp's force field (512 beads, LBS sites, dt = 1/2048)
+and for the benchmark's WCA + het LJ set (force cap, LAD wall, dt = 1e-3).
+
+Design: R = 16 replicas per set, each started from its own relaxed walk. The
+GPU and the oracle run replica k from the same start with different noise
+keys, so each pair follows the same stochastic process and differs only by
+its noise (and FP32 against FP64). For every observable (Rg^2, the 8-shell
+radial density, the contact probability P(s) at s = 1, 4, 16, 64) the time
+average of each run gives a paired difference d_k = gpu_k - oracle_k; under
+the hypothesis of equal dynamics the d_k are independent with mean zero, and
+t = mean(d) / (sd(d) / sqrt(R)) follows Student's t with R - 1 = 15 degrees
+of freedom. The bar |t| < 5 has a false-alarm probability of 1.6e-4 per
+observable (2e-3 over the 13 of a set), and a systematic difference of five
+standard errors of the paired mean fails it."""
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
 import pyoracle as po
-import sostat
 from gpu_common import make_pair, step
 from helpers import lbs_sites, pattern, radius_for, relaxed
+from tfm_amd.ccp import sequence_pattern
 
 pytestmark = pytest.mark.gpu
 
-N, STEPS, EVERY = 512, 160_000, 500
+R_REP, STEPS, EVERY = 16, 32_000, 200
 SEPS = (1, 4, 16, 64)
+NAMES = ["rg2"] + [f"rho{k}" for k in range(8)] + [f"P({s})" for s in SEPS]
+T_BAR = 5.0
 
 
 def observe(r, R_n):
     r = np.asarray(r, np.float64)
+    N = r.shape[0]
     cm = r.mean(axis=0)
     rg2 = ((r - cm) ** 2).sum(axis=1).mean()
     d = np.linalg.norm(r, axis=1)
@@ -30,33 +45,75 @@ def observe(r, R_n):
     return np.concatenate([[rg2], rho, ps])
 
 
-def test_long_run_observables_agree_with_oracle():
-    step(f"long run N={N} {STEPS} steps")
+def paired_replicas(N, R_n, fl, make, label):
+    """make(k, seed) -> (sim, osys, r64) for replica k; returns the t values."""
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    pool = ThreadPoolExecutor(max_workers=workers)
+
+    def oracle_run(osys, r):
+        out = []
+        for k in range(STEPS // EVERY):
+            r = osys.run(r, k * EVERY, EVERY, prec=64, threads=1)
+            out.append(observe(r, R_n))
+        return np.mean(out, axis=0)
+
+    futures, gpu = [], []
+    for k in range(R_REP):
+        sim, osys, r64 = make(k)
+        futures.append(pool.submit(oracle_run, osys, r64.copy()))
+        sim.set_time(0, 0.0, 0)
+        obs = []
+        for _ in range(STEPS // EVERY):
+            sim.run_steps(EVERY)
+            obs.append(observe(sim.positions, R_n))
+        sim.close()
+        gpu.append(np.mean(obs, axis=0))
+        step(f"  {label}: replica {k} on the GPU done")
+    orc = np.array([f.result() for f in futures])
+    pool.shutdown()
+    gpu = np.array(gpu)
+    d = gpu - orc
+    t = d.mean(axis=0) / np.maximum(d.std(axis=0, ddof=1) / np.sqrt(R_REP), 1e-15)
+    for j, name in enumerate(NAMES):
+        step(f"  {label} {name}: gpu {gpu[:, j].mean():.4f} oracle {orc[:, j].mean():.4f} "
+             f"paired diff {d[:, j].mean():+.2e} +- {d[:, j].std(ddof=1) / np.sqrt(R_REP):.1e}  t {t[j]:+.2f}")
+    return t
+
+
+def test_long_run_observables_agree_with_oracle_ccp_set():
+    N = 512
+    step(f"long run, ccp set, N={N}, {R_REP} paired replicas x {STEPS} steps")
     R_n = radius_for(N, 0.2)
-    r0 = relaxed(N, R_n, 31)
     fl = pattern(N, 5)
     lbs = lbs_sites(R_n, 20, 4)
-    sim, _, r64 = make_pair(N, R_n, fl, r0, lbs=lbs, seed=101)
-    osys = make_pair(N, R_n, fl, r0, lbs=lbs, seed=202)[1]  # same system, another noise key
-    sim.set_time(0, 0.0, 0)
-    g, o = [], []
-    ro = r64.copy()
-    for k in range(STEPS // EVERY):
-        sim.run_steps(EVERY)
-        g.append(observe(sim.positions, R_n))
-        ro = osys.run(ro, k * EVERY, EVERY, prec=64, threads=8)
-        o.append(observe(ro, R_n))
-    sim.close()
-    g, o = np.array(g, np.float32), np.array(o, np.float32)
-    names = ["rg2"] + [f"rho{k}" for k in range(8)] + [f"P({s})" for s in SEPS]
-    worst = 0.0
-    for j, name in enumerate(names):
-        a, _, sa, _, _, _ = sostat.series(g[:, j])
-        b, _, sb, _, _, _ = sostat.series(o[:, j])
-        z = abs(a - b) / max(np.hypot(sa, sb), 1e-12)
-        worst = max(worst, z)
-        step(f"  {name}: gpu {a:.4f} +- {sa:.4f}  oracle {b:.4f} +- {sb:.4f}  z {z:.2f}")
-        # the blocking error underestimates the spread of the slowest
-        # observables (P(64) relaxes on the run's own time scale), hence 6
-        assert z < 6.0, name
-    step(f"  largest z {worst:.2f}")
+
+    def make(k):
+        r0 = relaxed(N, R_n, 31 + k)
+        sim, _, r64 = make_pair(N, R_n, fl, r0, lbs=lbs, seed=101 + k)
+        osys = make_pair(N, R_n, fl, r0, lbs=lbs, seed=5101 + k)[1]  # same start, another noise key
+        return sim, osys, r64
+
+    t = paired_replicas(N, R_n, fl, make, "ccp")
+    assert np.abs(t).max() < T_BAR, dict(zip(NAMES, np.round(t, 2)))
+
+
+def test_long_run_observables_agree_with_oracle_lj_set():
+    # the benchmark's physics: WCA for every pair, het-het LJ eps 0.6 to
+    # 2.5 sigma, LAD beads attracted by the lamina well, force cap, dt 1e-3,
+    # on the eight-lane kernel over half-size cells (the benchmark's shape)
+    N = 384
+    step(f"long run, LJ set, N={N}, {R_REP} paired replicas x {STEPS} steps")
+    R_n = radius_for(N, 0.25)
+    # the het/LAD generator's pattern: 30 % het in Geometric(20) blocks, 15 % LAD at the chain ends
+    fl = sequence_pattern({"number_of_particles": N, "nucleus_radius": R_n, "force_field": "lj", "seed": 11,
+                           "het_fraction": 0.3, "lad_fraction": 0.15}, N)
+
+    def make(k):
+        r0 = relaxed(N, R_n, 61 + k)
+        kw = dict(ff=po.FF_LJ, dt=1e-3, eps_het=0.6, eps_lam=1.0)
+        sim, _, r64 = make_pair(N, R_n, fl, r0, seed=301 + k, lanes_per_bead=8, cell_reach=2, **kw)
+        osys = make_pair(N, R_n, fl, r0, seed=7301 + k, **kw)[1]
+        return sim, osys, r64
+
+    t = paired_replicas(N, R_n, fl, make, "lj")
+    assert np.abs(t).max() < T_BAR, dict(zip(NAMES, np.round(t, 2)))

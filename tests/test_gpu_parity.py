@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
+import tfm_amd
 from gpu_common import make_pair, step
 from helpers import lbs_sites, pattern, radius_for, relaxed, walk
 
@@ -192,4 +193,87 @@ def test_overlap_count_matches_brute_force():
     iu = np.triu_indices(N, k=2)
     want = int((d[iu] < 0.838732).sum())
     assert sim.count_overlaps(0.838732) == want
+    sim.close()
+
+
+def test_step_grid_bit_exact():
+    # The step's own cell list (1.25 sigma cells, lookup by multiplying with
+    # 1/cell_size, binned in the force kernel's epilogue and ordered by the
+    # order pass) against a stable sort by the same rule, on the benchmark's
+    # workload: after the initial build and after steps.
+    N = 131072
+    R_n = radius_for(N, 0.3)
+    step(f"step grid N={N}")
+    sim = tfm_amd.Simulation({"number_of_particles": N, "nucleus_radius": repr(float(R_n)), "number_of_chains": 8,
+                              "force_field": "lj", "epsilon_het": 0.6, "lad_fraction": 0.15, "seed": 3,
+                              "timestep": 0.001, "lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.25})
+    sim.generate_initial_condition()
+    info = sim.info
+    cps, csl = info.fine_cps, np.float32(info.fine_csl)
+    icsl = np.float32(1.0) / csl
+    for label, n in (("initial build", 0), ("after 7 steps", 7)):
+        if n:
+            sim.run_steps(n)
+        cs, soi = sim.step_grid()
+        r = sim.positions
+        g = np.floor(r * icsl).astype(np.int64) + cps // 2
+        assert g.min() >= 0 and g.max() < cps
+        key = (g[:, 2] * cps + g[:, 1]) * cps + g[:, 0]
+        order = np.argsort(key, kind="stable")
+        start = np.concatenate([[0], np.cumsum(np.bincount(key, minlength=cps ** 3))]).astype(np.uint32)
+        step(f"  {label}: {np.count_nonzero(np.diff(start))} occupied cells")
+        assert np.array_equal(cs, start), label
+        assert np.array_equal(soi, order.astype(np.uint32)), label
+    sim.close()
+
+
+@pytest.mark.parametrize("reach", [2, 1])
+def test_forces_crowded_lists(reach):
+    # Het beads on a cubic lattice of spacing 1 sigma (volume fraction 0.52):
+    # every bead's trimmed neighbour rows hold more slots than the eight-lane
+    # kernel's 128-entry list, so each lane scans its own rows directly (the
+    # crowded path), at reach 2 (1.25 sigma cells) and reach 1 (2.5 sigma).
+    step(f"forces, crowded lists, reach {reach}")
+    k = 12
+    N = k ** 3
+    rng = np.random.default_rng(5)
+    idx = np.arange(N)
+    x, y, z = idx % k, (idx // k) % k, idx // (k * k)
+    # a snake through the lattice, so consecutive beads are one spacing apart
+    x = np.where(y % 2 == 1, k - 1 - x, x)
+    y = np.where(z % 2 == 1, k - 1 - y, y)
+    r = np.stack([x, y, z], axis=1).astype(np.float64) - (k - 1) / 2 + rng.uniform(-0.05, 0.05, (N, 3))
+    R_n = 14.0
+    fl = np.full(N, po.HET, np.uint8)
+    sim, osys, r64 = make_pair(N, R_n, fl, r, ff=po.FF_LJ, eps_het=0.6, eps_lam=1.0, cell_reach=reach,
+                               lanes_per_bead=8)
+    # list length of every bead on the host: the slots of the cells whose
+    # nearest point lies within its cutoff (2.5 sigma + 1e-3)
+    csl = np.float32(sim.info.fine_csl)
+    cps = sim.info.fine_cps
+    r32 = r64.astype(np.float32)
+    g = np.floor(r32 * (np.float32(1.0) / csl)).astype(np.int64) + cps // 2
+    occ = {}
+    for c in map(tuple, g):
+        occ[c] = occ.get(c, 0) + 1
+    lo = (g - cps // 2) * csl
+    shortest = None
+    for i in range(0, N, 97):
+        tot = 0
+        for (cx, cy, cz), m in occ.items():
+            gap = 0.0
+            for a, c in enumerate((cx, cy, cz)):
+                e0 = (c - cps // 2) * csl
+                d = max(e0 - r32[i, a], r32[i, a] - (e0 + csl), 0.0)
+                gap += d * d
+            if gap <= (2.5 + 1e-3) ** 2:
+                tot += m
+        shortest = tot if shortest is None else min(shortest, tot)
+    assert shortest > 128, shortest
+    got = sim.compute_forces(po.FF_LJ).astype(np.float64)
+    want = osys.forces(r64, method=0)
+    w32 = osys.forces(r64, method=0, prec=32)
+    err, err32, scale = np.abs(got - want).max(), np.abs(w32 - want).max(), np.abs(want).max()
+    step(f"  shortest list {shortest}, max|f|={scale:.3g} gpu err={err:.3g} oracle fp32 err={err32:.3g}")
+    assert err <= 3 * err32 + 1e-6 * scale
     sim.close()

@@ -1,5 +1,5 @@
-"""Build recipe for libccpmi.so (gfx950) and the ccp-perform / ccp-analyze
-drop-in executables.
+"""Build recipe for libccpmi.so (gfx950) and the ccp-perform / ccp-analyze /
+ccp-profile drop-in executables.
 
 Everything is compiled in-tree with hipcc so the built files travel with the
 source tree.
@@ -40,7 +40,8 @@ def build_lib():
 
 def build_cli():
     os.makedirs(BIN, exist_ok=True)
-    for name, src in (("ccp-perform", "ccp_perform.cpp"), ("ccp-analyze", "ccp_analyze.cpp")):
+    for name, src in (("ccp-perform", "ccp_perform.cpp"), ("ccp-analyze", "ccp_analyze.cpp"),
+                      ("ccp-profile", "ccp_profile.cpp")):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue

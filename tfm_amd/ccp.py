@@ -28,6 +28,7 @@ EXPORTS = (
     "ccp_get_types", "ccp_set_types", "ccp_get_flags", "ccp_set_flags", "ccp_get_lbs", "ccp_set_lbs",
     "ccp_get_chains", "ccp_compute_forces", "ccp_cell_list", "ccp_noise", "ccp_count_overlaps",
     "ccp_contact_sample", "ccp_contact_reset", "ccp_contact_map", "ccp_time_kernels", "ccp_capture_graphs",
+    "ccp_step_grid",
     "ccp_observables",
     "ccp_distance_contact", "ccp_frame_contact_map", "ccp_analysis_create", "ccp_analysis_destroy",
     "ccp_analysis_add_frame", "ccp_analysis_add_trajectory_file", "ccp_analysis_end_simulation",
@@ -115,6 +116,7 @@ def lib():
         L.ccp_contact_map.argtypes = [vp, u32p, sz, u32p]
         L.ccp_time_kernels.argtypes = [vp, ctypes.c_uint32, f32p]
         L.ccp_capture_graphs.argtypes = [vp]
+        L.ccp_step_grid.argtypes = [vp, u32p, sz, u32p, sz]
         f64p = P(ctypes.c_double)
         L.ccp_observables.argtypes = [vp, f64p, f64p, sz]
         L.ccp_distance_contact.argtypes = [vp, ctypes.c_uint32, f32p, f32p, sz]
@@ -351,6 +353,15 @@ class Simulation:
         out = np.zeros(5, np.float32)
         self._check(lib().ccp_time_kernels(self.h, n_steps, _ptr(out, ctypes.c_float)))
         return out
+
+    def step_grid(self):
+        """The step's own cell list of the current positions: (cstart, soi)."""
+        fc = self.info.fine_cps
+        cs = np.zeros(fc ** 3 + 1, np.uint32)
+        soi = np.zeros(self.N, np.uint32)
+        self._check(lib().ccp_step_grid(self.h, _ptr(cs, ctypes.c_uint32), cs.size, _ptr(soi, ctypes.c_uint32),
+                                        self.N))
+        return cs, soi
 
     def capture_graphs(self):
         """Capture every step block run_steps replays, ahead of a timed run."""

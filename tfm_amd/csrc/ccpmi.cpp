@@ -228,10 +228,11 @@ void ccp_sim::fill_kparams() {
   k.e_p = E_P;
   k.rco = RCO;
   k.rco2 = RCO * RCO;
+  k.aco = ACO;
   k.aco2 = ACO * ACO;
-  k.lfac = E_L * (8.0f / (3.0f * LCO * LCO));
-  k.lco2 = LCO * LCO;
-  k.inv_lco2 = 1.0f / (LCO * LCO);
+  // ccp's e_l * (8.0 / (3.0 * lco * lco)): float constants, double literals
+  k.lfac_d = (double)E_L * (8.0 / (3.0 * (double)LCO * (double)LCO));
+  k.lco = LCO;
   k.eps_wca = 1.0f;
   k.rm2 = (float)std::pow(2.0, 1.0 / 3.0);
   k.eps_het = eps_het;
@@ -1223,6 +1224,25 @@ int ccp_cell_list(ccp_sim *h, uint32_t *sci, uint32_t *soi, uint32_t *beg, uint3
       beg[c] = full ? cs[c] : 0u;
       end[c] = full ? cs[c + 1] : 0u;
       for (uint32_t s = cs[c]; s < cs[c + 1]; ++s) sci[s] = (uint32_t)c;
+    }
+  });
+}
+
+int ccp_step_grid(ccp_sim *h, uint32_t *cstart, size_t n_c1, uint32_t *soi, size_t n) {
+  return guarded(h, [&] {
+    need(cstart && soi && n == h->N && n_c1 == (size_t)h->gf.g.n_c + 1, "step grid size mismatch");
+    // the grid of the current positions: the one the last stage binned and
+    // ordered, or a fresh build when the positions were set since
+    h->ensure_grids();
+    std::vector<float4> sp(h->N);
+    hip_check(hipMemcpyAsync(cstart, h->gf.cstart.p, 4 * n_c1, hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipMemcpyAsync(sp.data(), h->gf.spos.p, sizeof(float4) * n, hipMemcpyDeviceToHost, h->stream));
+    hip_check(hipStreamSynchronize(h->stream));
+    h->check_flags();
+    for (size_t k = 0; k < n; ++k) {
+      uint32_t w;
+      std::memcpy(&w, &sp[k].w, 4);
+      soi[k] = w & ID_MASK;
     }
   });
 }

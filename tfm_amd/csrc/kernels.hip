@@ -42,6 +42,31 @@ namespace {
 __device__ __forceinline__ uint32_t wbits(float4 v) { return __float_as_uint(v.w); }
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// ccp's Wang-Frenkel factor e (18 d^4 - 96 d^2 + 96) / d^8 (calc_ppf<DST>,
+// chrsim.cu:173-185; calc_wpf, :79-88, with e = 1) in ccp's arithmetic: d is
+// the FP32 length, d^2 its FP32 square, the polynomial and the division in
+// FP64 (ccp's double literals promote them), d^8 the FP32 product
+// ((d^2 d^2) d^2) d^2, and the factor rounded to FP32 before it scales the
+// vector (vect.cuh's operator*(float, vec3f)).
+__device__ __forceinline__ float ccp_wf(float e, float d) {
+#pragma clang fp contract(off)
+  const float d2 = d * d;
+  const double x = (double)d2;
+  const double poly = 18.0 * x * x - 96.0 * x + 96.0;
+  const float d8 = d2 * d2 * d2 * d2;
+  return (float)((double)e * poly / (double)d8);
+}
+
+// ccp's binding factor e_l 8/(3 lco^2) ((d/lco)^6 - 1) (calc_lpf<DST>,
+// chrsim.cu:201-212): nd = d/lco and its square in FP32, the cube of the
+// square in FP32, the rest in FP64 (lfac = e_l 8/(3 lco^2) in FP64).
+__device__ __forceinline__ float ccp_lbs(double lfac, float d, float lco) {
+#pragma clang fp contract(off)
+  const float nd = d / lco;
+  const float d2 = nd * nd;
+  return (float)(lfac * ((double)(d2 * d2 * d2) - 1.0));
+}
 // the element at byte offset `off` of `base` (a 32-bit offset: the load takes
 // the scalar base and the offset register as they are)
 template <class T> __device__ __forceinline__ T at_byte(const T *base, uint32_t off) {
@@ -214,9 +239,8 @@ __device__ __forceinline__ void pair_short(const KParams &p, float rx, float ry,
   const bool nb = (j > i + 1u) || (i > j + 1u);
   float fac;
   if constexpr (MODE == MODE_CCP) {
-    const float u = fast_rcp(d2);
-    fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
-    fac = (nb && d2 <= p.rco2) ? fac : 0.0f;
+    const float d = sqrtf(d2);
+    fac = (nb && d <= p.rco) ? ccp_wf(p.e_p, d) : 0.0f;
   } else if constexpr (MODE == MODE_ICG) {
     const float d = sqrtf(d2);
     fac = 128.0f * (3.0f * p.rco - 3.0f * d);
@@ -244,12 +268,12 @@ __device__ __forceinline__ void pair_tail(const KParams &p, float rx, float ry, 
   const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
   const float d2 = dx * dx + dy * dy + dz * dz;
   const bool nb = (j > i + 1u) || (i > j + 1u);
-  const float u = fast_rcp(d2);
   float fac;
   if constexpr (MODE == MODE_CCP) {
-    fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
-    fac = (nb && d2 > p.rco2 && d2 <= p.aco2) ? fac : 0.0f;
+    const float d = sqrtf(d2);
+    fac = (nb && d > p.rco && d <= p.aco) ? ccp_wf(p.e_p, d) : 0.0f;
   } else {
+    const float u = fast_rcp(d2);
     const float s2 = u;
     const float s6 = s2 * s2 * s2;
     fac = p.eps_het * 24.0f * s6 * (2.0f * s6 - 1.0f) * u;
@@ -264,10 +288,8 @@ __device__ __forceinline__ void pair_tail(const KParams &p, float rx, float ry, 
 __device__ __forceinline__ void lbs_force(const KParams &p, float rx, float ry, float rz, float4 o, float &fx,
                                           float &fy, float &fz) {
   const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
-  const float d2 = dx * dx + dy * dy + dz * dz;
-  const float n2 = d2 * p.inv_lco2;
-  float fac = p.lfac * (n2 * n2 * n2 - 1.0f);
-  fac = (d2 <= p.lco2) ? fac : 0.0f;
+  const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+  const float fac = d <= p.lco ? ccp_lbs(p.lfac_d, d, p.lco) : 0.0f;
   fx += fac * dx;
   fy += fac * dy;
   fz += fac * dz;
@@ -390,9 +412,9 @@ __device__ __forceinline__ void wall_force(const KParams &p, float rx, float ry,
   if (!isfinite(dw2)) return;
   float fac;
   if constexpr (MODE != MODE_LJ) {
-    if (dw2 > p.rco2) return;
-    const float u = fast_rcp(dw2);
-    fac = u * u * (18.0f + u * (96.0f * u - 96.0f));
+    const float d = sqrtf(dw2);
+    if (d > p.rco) return;
+    fac = ccp_wf(1.0f, d);
   } else {
     const float u = fast_rcp(dw2);
     const float s2 = u, s6 = s2 * s2 * s2;
@@ -424,9 +446,8 @@ __device__ __forceinline__ void pair_full(const KParams &p, float rx, float ry, 
   const bool hhi = (wi & wj & W_HET) != 0u;
   float fac;
   if constexpr (MODE == MODE_CCP) {
-    const float u = fast_rcp(d2);
-    fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
-    fac = (nb && d2 <= (hhi ? p.aco2 : p.rco2)) ? fac : 0.0f;
+    const float d = sqrtf(d2);
+    fac = (nb && d <= (hhi ? p.aco : p.rco)) ? ccp_wf(p.e_p, d) : 0.0f;
   } else if constexpr (MODE == MODE_ICG) {
     const float d = sqrtf(d2);
     fac = 128.0f * (3.0f * p.rco - 3.0f * d);
@@ -482,9 +503,8 @@ __device__ __forceinline__ void pair_term(const KParams &p, const PairCtx &c, fl
   const bool hhi = (wj & c.het) != 0u;
   float fac;
   if constexpr (MODE == MODE_CCP) {
-    const float u = fast_rcp(d2);
-    fac = p.e_p * u * u * (18.0f + u * (96.0f * u - 96.0f));
-    fac = (nb & (d2 <= (hhi ? p.aco2 : p.rco2))) ? fac : 0.0f;
+    const float d = sqrtf(d2);
+    fac = (nb & (d <= (hhi ? p.aco : p.rco))) ? ccp_wf(p.e_p, d) : 0.0f;
   } else if constexpr (MODE == MODE_ICG) {
     const float d = sqrtf(d2);
     fac = 128.0f * (3.0f * p.rco - 3.0f * d);

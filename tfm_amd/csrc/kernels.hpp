@@ -80,9 +80,10 @@ struct KParams {
   int bend_all, bond_type;
   float fene_k, fene_r02;
   // Wang-Frenkel pair set (ccp)
-  float e_p, rco, rco2, aco2;
-  // LBS binding
-  float lfac, lco2, inv_lco2;
+  float e_p, rco, rco2, aco, aco2;
+  // LBS binding: lfac_d = e_l 8/(3 lco^2) in FP64 as ccp evaluates it
+  double lfac_d;
+  float lco;
   // WCA/LJ set in units of sigma (sigma = 1)
   float eps_wca, rm2, eps_het, rc_lj2, eps_wall, eps_lam, rc_lam2;
   // squared candidate-list radii of the balanced kernel: cutoff + 1e-3 sigma
