@@ -172,6 +172,22 @@ int ccp_cell_list(ccp_sim *h, uint32_t *sci, uint32_t *soi, uint32_t *beg, uint3
  * sees them. */
 int ccp_step_grid(ccp_sim *h, uint32_t *cstart, size_t n_c1, uint32_t *soi, size_t n);
 
+/* The cell list of ccp_cell_list for a device position array, with no
+ * device allocation: xyz is n interleaved float triples in device memory,
+ * csl and cps ccp's cell side and cells per side (sugrid(N, csl, cps),
+ * sugrid.cuh:15-54); sci/soi (n entries) and beg/end (cps^3 entries) are
+ * device arrays that receive ccp's layout, bit for bit what ccp_cell_list
+ * gives for the same positions and geometry. Launches on `stream` (a
+ * hipStream_t; NULL is the null stream) and waits for it. No counterpart
+ * in ccp: it serves callers that cannot allocate device memory (code under
+ * hipGraph capture, serving loops, harnesses that forbid hipMalloc). Its
+ * only device memory beyond the arguments is a fixed array of 256 KiB per
+ * GPU, so cps is at most 255 (CCP_ERR_ARG beyond); calls that overlap in
+ * time on one GPU share that array and race. A bead outside the grid gives
+ * CCP_ERR_STATE. Errors: ccp_last_error(NULL). */
+int ccp_cell_list_noalloc(const float *xyz, uint32_t n, float csl, uint32_t cps, uint32_t *sci, uint32_t *soi,
+                          uint32_t *beg, uint32_t *end, void *stream);
+
 /* The noise of step `step` for every bead, in units of one standard
  * deviation (begin_iter, chrsim.cu:348-356), and the redraw count. */
 int ccp_noise(ccp_sim *h, uint64_t step, float *z_xyz, uint32_t *attempts, size_t n);

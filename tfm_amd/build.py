@@ -32,7 +32,7 @@ def _run(cmd):
 
 def build_lib():
     srcs = [os.path.join(CSRC, "kernels.hip"), os.path.join(CSRC, "ccpmi.cpp"), os.path.join(CSRC, "batch.cpp"),
-            os.path.join(CSRC, "nn.hip")]
+            os.path.join(CSRC, "nn.hip"), os.path.join(CSRC, "noalloc.hip")]
     tmp = LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, LIB)

@@ -28,7 +28,7 @@ EXPORTS = (
     "ccp_get_types", "ccp_set_types", "ccp_get_flags", "ccp_set_flags", "ccp_get_lbs", "ccp_set_lbs",
     "ccp_get_chains", "ccp_compute_forces", "ccp_cell_list", "ccp_noise", "ccp_count_overlaps",
     "ccp_contact_sample", "ccp_contact_reset", "ccp_contact_map", "ccp_time_kernels", "ccp_capture_graphs",
-    "ccp_step_grid",
+    "ccp_cell_list_noalloc", "ccp_step_grid",
     "ccp_observables",
     "ccp_distance_contact", "ccp_frame_contact_map", "ccp_analysis_create", "ccp_analysis_destroy",
     "ccp_analysis_add_frame", "ccp_analysis_add_trajectory_file", "ccp_analysis_end_simulation",
@@ -117,6 +117,8 @@ def lib():
         L.ccp_time_kernels.argtypes = [vp, ctypes.c_uint32, f32p]
         L.ccp_capture_graphs.argtypes = [vp]
         L.ccp_step_grid.argtypes = [vp, u32p, sz, u32p, sz]
+        L.ccp_cell_list_noalloc.argtypes = [f32p, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, u32p, u32p, u32p,
+                                            u32p, vp]
         f64p = P(ctypes.c_double)
         L.ccp_observables.argtypes = [vp, f64p, f64p, sz]
         L.ccp_distance_contact.argtypes = [vp, ctypes.c_uint32, f32p, f32p, sz]

@@ -823,6 +823,11 @@ ccp_sim *ccp_create_logged(const char *param_text, ccp_log_fn fn, void *user) {
 void ccp_destroy(ccp_sim *h) { delete h; }
 
 const char *ccp_last_error(const ccp_sim *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+} // extern "C"
+
+void ccpmi::host::set_null_error(const std::string &msg) { g_create_error = msg; }
+
+extern "C" {
 
 int ccp_set_logger(ccp_sim *h, ccp_log_fn fn, void *user) {
   return guarded(h, [&] {

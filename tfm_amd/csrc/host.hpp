@@ -44,6 +44,9 @@ inline void hip_check(hipError_t e) {
   if (e != hipSuccess) throw Error(CCP_ERR_HIP, std::string("hip: ") + hipGetErrorString(e));
 }
 
+// the message ccp_last_error(NULL) returns on this thread (calls with no handle)
+void set_null_error(const std::string &msg);
+
 // whitespace "key value" pairs; the first occurrence of a key wins (util.cu:72-80)
 struct ParMap : std::map<std::string, std::string> {
   explicit ParMap(const char *text) {
