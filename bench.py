@@ -77,8 +77,15 @@ BYTES_ORDER_PER_CELL = 12
 # state the timed steps start from: the initial condition advanced by this
 # many untimed steps (the relaxed walk's first steps are slower than the
 # long run's)
-SETTLE_STEPS = 3000
+SETTLE_STEPS = 20000
 SETTLE_SHAPE = {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.25}
+
+
+def sim_shape(params, shape):
+    """The kernel-shape keys the timed simulation runs with."""
+    out = {k: params[k] for k in ("lanes_per_bead", "cell_reach", "cell_size") if k in params}
+    out.update(shape)
+    return out
 
 
 def parse():
@@ -90,8 +97,8 @@ def parse():
     ap.add_argument("--settle", type=int, default=SETTLE_STEPS,
                     help="untimed steps from the initial condition to the state the run starts from")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_force_lj_n131072_traffic.txt"))
-    ap.add_argument("--sq", default=os.path.join(ROOT, "profiles", "k_force_bal_lj_n131072_sq.txt"))
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_force_typed_lj_n131072_traffic.txt"))
+    ap.add_argument("--sq", default=os.path.join(ROOT, "profiles", "k_force_typed_lj_n131072_sq.txt"))
     ap.add_argument("--param", action="append", default=[], metavar="KEY=VALUE",
                     help="override a simulation parameter (e.g. fine_cell_size=1.25)")
     ap.add_argument("--no-tune", action="store_true", help="skip the kernel-shape autotuner")
@@ -249,7 +256,10 @@ def main():
     kt = sim.time_kernels(200)
     stage = 1 if kt[0] >= kt[2] else 2
     force_ms = float(kt[0] if stage == 1 else kt[2])
-    kname = "k_force_bal" if shape.get("lanes_per_bead") == 8 else "k_force"
+    eight = int(sim_shape(params, shape).get("lanes_per_bead", 0)) == 8
+    reach2 = int(sim_shape(params, shape).get("cell_reach", 1)) == 2
+    typed = eight and reach2 and str(params.get("type_order", 1)) != "0"
+    kname = "k_force_typed" if typed else "k_force_bal" if eight else "k_force"
     dominant = f"{kname}<LJ,{stage}>"
     bytes_force = BYTES_FORCE_PER_BEAD[stage] * N
     achieved = bytes_force / (force_ms * 1e-3) / 1e9
@@ -284,7 +294,7 @@ def main():
             "overrides": args.param,
         },
         "roofline": {
-            "bound": "hbm", "limiter": "valu issue (see profiles/k_force_bal_lj_n131072_sq.txt)", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "bound": "hbm", "limiter": "valu issue and load latency (see profiles/k_force_typed_lj_n131072_sq.txt)", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             # whole step at 176 algorithmic bytes per bead-step (index-array cell list)
             "step_frac_176B": 176.0 * value / (HBM_PEAK_GBS * 1e9),

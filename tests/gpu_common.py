@@ -24,7 +24,8 @@ def make_pair(N, R_n, flags, r, ff=po.FF_CCP, lbs=None, chain_off=None, seed=9, 
         okw.update(bend_all=1, k_b=2.0, eps_het=kw.get("eps_het", 0.5), eps_lam=kw.get("eps_lam", 1.0))
     if dt is not None:
         params["timestep"] = repr(float(dt))
-    for k in ("opening_radius", "bleb_radius", "lanes_per_bead", "fine_cell_size", "cell_reach", "integrate_pass", "group_types", "bal_rounds"):
+    for k in ("opening_radius", "bleb_radius", "lanes_per_bead", "fine_cell_size", "cell_reach", "integrate_pass", "group_types", "bal_rounds",
+              "type_order", "cell_size"):
         if k in kw:
             params[k] = kw[k]
     if chain_off is not None:

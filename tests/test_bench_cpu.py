@@ -1,7 +1,7 @@
 """bench.py reads its traffic and vector-ALU figures from the committed
 rocprofv3 summaries under profiles/: they parse, and the figures are
-plausible for the benchmark's force kernel (N = 131072, 88 algorithmic bytes
-per bead in stage 1, 16384 wavefronts per launch)."""
+plausible for the benchmark's force kernel (N = 131072, 100 algorithmic bytes
+per bead in stage 1, between 8 and 16 beads per wavefront)."""
 import importlib.util
 import os
 
@@ -17,16 +17,17 @@ def load_bench():
 
 def test_profile_summaries_parse():
     b = load_bench()
-    traffic = b.read_traffic(os.path.join(ROOT, "profiles", "k_force_lj_n131072_traffic.txt"))
+    assert b.BYTES_FORCE_PER_BEAD == {1: 100, 2: 84}
+    traffic = b.read_traffic(os.path.join(ROOT, "profiles", "k_force_typed_lj_n131072_traffic.txt"))
     algorithmic = b.BYTES_FORCE_PER_BEAD[1] * b.N
     # measured HBM bytes of a launch: at least the algorithmic bytes, and the
     # write-back of scattered bucket entries keeps it within a few times that
     assert traffic is not None and algorithmic <= traffic <= 4 * algorithmic
-    valu = b.read_valu(os.path.join(ROOT, "profiles", "k_force_bal_lj_n131072_sq.txt"))
+    valu = b.read_valu(os.path.join(ROOT, "profiles", "k_force_typed_lj_n131072_sq.txt"))
     assert valu is not None
     insts, us = valu
-    waves = b.N // 8  # eight beads per wavefront
-    assert 300 * waves <= insts <= 3000 * waves
+    # eight beads per wavefront (het beads) to sixteen (the others)
+    assert 300 * (b.N // 16) <= insts <= 3000 * (b.N // 8)
     # the issue rate those imply stays below the 614.4 G/s peak
     assert 0.0 < insts / (us * 1e-6) / 1e9 < b.VALU_PEAK_GWIPS
 

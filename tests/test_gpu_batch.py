@@ -193,13 +193,15 @@ def test_ensemble_full_size():
     assert all(np.array_equal(m, m.T) for m in maps)
     assert (maps.reshape(256, -1).sum(axis=1) > 0).all()
     # each system's mean radial density integrates to its beads' volume
-    # (less the small part of wall beads' spheres outside the nucleus)
+    # (less the small part of wall beads' spheres outside the nucleus); the
+    # shell volume here comes from the radius recomputed in double, which can
+    # differ from the library's FP32 radius in its last place (3e-7 in R^3)
     vb = 4.0 / 3.0 * np.pi * (float(np.float32(1.154701)) / 2) ** 3
     for s in range(256):
         sim_R = float(np.float32(float(f"{0.5773505 + 0.5773505 * (lengths[s] / systems[s]['volume_fraction']) ** (1 / 3):.6f}")))
         shell = 4.0 / 3.0 * np.pi * sim_R ** 3 / 32
         tot = rd[s].sum() * shell / (lengths[s] * vb)
-        assert 0.95 < tot <= 1.0 + 1e-9, (s, tot)
+        assert 0.95 < tot <= 1.0 + 1e-6, (s, tot)
     for s in check:
         np.testing.assert_allclose(rd[s], want_rd[s] / 4, rtol=1e-9, atol=1e-12)
     # four systems (the shortest, the longest, two others) alone, bit for bit

@@ -264,24 +264,33 @@ def test_full_size_workload_properties():
 def test_type_grouping_is_bit_exact():
     # the het-first reordering of each workgroup's beads moves beads between
     # wavefronts but keeps every bead's candidate list and summation order:
-    # trajectories of the benchmark workload must agree to the bit
+    # trajectories of the benchmark workload must agree to the bit (the
+    # single kernel for every bead, without the type-ordered work list)
     step("type grouping bit-exact N=131072")
     import bench
 
-    shape = {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.25}
+    shape = {"lanes_per_bead": 8, "cell_reach": 2, "cell_size": 1.25, "type_order": 0}
     a = tfm_amd.Simulation({**bench.PARAMS, **shape, "group_types": 1})
     a.generate_initial_condition()
     xyz, fl = a.positions, a.flags
     b = tfm_amd.Simulation({**bench.PARAMS, **shape, "group_types": 0})
     b.flags = fl
     b.positions = xyz
-    for s in (a, b):
+    # the type-ordered work list: the het beads keep the same lists and sums,
+    # the others run on four lanes over reach 1, another FP32 summation order
+    c = tfm_amd.Simulation({**bench.PARAMS, **shape, "type_order": 1})
+    c.flags = fl
+    c.positions = xyz
+    for s in (a, b, c):
         s.set_time(0, 0.0, 0)
         s.run_steps(300)
         s.synchronize()
     assert np.array_equal(a.positions, b.positions)
-    a.close()
-    b.close()
+    dev = np.abs(c.positions.astype(np.float64) - a.positions).max()
+    step(f"  work list vs one kernel after 300 steps: max deviation {dev:.3g}")
+    assert dev < 1e-3
+    for s in (a, b, c):
+        s.close()
 
 
 def test_run_frames_callback():

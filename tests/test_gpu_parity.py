@@ -63,8 +63,15 @@ FORCE_CASES = [
     # eight lanes for the pairs, bonds / wall / stage in a second one-lane-per-bead pass
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, integrate_pass=1),
     dict(ff=po.FF_CCP, n_l=40, lanes_per_bead=8, integrate_pass=1),
-    # eight lanes without the het-first reordering of each workgroup's beads
-    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, group_types=0),
+    # eight lanes at reach 2 without the type-ordered work list (one kernel
+    # for every bead, het beads first within each workgroup), and without
+    # that reordering either
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, type_order=0),
+    dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, type_order=0,
+         group_types=0),
+    # the type-ordered work list with ccp's force field (1.25 sigma cells keep
+    # its short cutoff within one cell)
+    dict(ff=po.FF_CCP, n_l=40, cell_reach=2, lanes_per_bead=8, cell_size=1.25),
     # eight lanes per bead, the pair phase of 2, 4 or 8 groups of eight beads
     # per wavefront before the per-bead work
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, bal_rounds=2),
@@ -132,12 +139,12 @@ def test_forces_partial_workgroup(rounds):
     sim.close()
 
 
-@pytest.mark.parametrize("ff,dt,tol,reach,lanes,rounds", [
-    (po.FF_CCP, 1 / 2048, 1e-3, 1, 3, 1), (po.FF_LJ, 1e-3, 2e-3, 1, 3, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 5, 1),
-    (po.FF_CCP, 1 / 2048, 1e-3, 1, 8, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 8, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 8, 4),
-    (po.FF_CCP, 1 / 2048, 1e-3, 1, 8, 8)])
-def test_trajectory_1000_steps(ff, dt, tol, reach, lanes, rounds):
-    step(f"trajectory ff={ff} reach={reach} lanes={lanes} rounds={rounds}")
+@pytest.mark.parametrize("ff,dt,tol,reach,lanes,rounds,typed", [
+    (po.FF_CCP, 1 / 2048, 1e-3, 1, 3, 1, 1), (po.FF_LJ, 1e-3, 2e-3, 1, 3, 1, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 5, 1, 1),
+    (po.FF_CCP, 1 / 2048, 1e-3, 1, 8, 1, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 8, 1, 1), (po.FF_LJ, 1e-3, 2e-3, 2, 8, 1, 0),
+    (po.FF_LJ, 1e-3, 2e-3, 2, 8, 4, 1), (po.FF_CCP, 1 / 2048, 1e-3, 1, 8, 8, 1)])
+def test_trajectory_1000_steps(ff, dt, tol, reach, lanes, rounds, typed):
+    step(f"trajectory ff={ff} reach={reach} lanes={lanes} rounds={rounds} type_order={typed}")
     N = 1024
     R_n = radius_for(N, 0.2)
     r = relaxed(N, R_n, 2)
@@ -149,7 +156,7 @@ def test_trajectory_1000_steps(ff, dt, tol, reach, lanes, rounds):
         fl = pattern(N, 1)
         lbs = lbs_sites(R_n, 30, 8)
     sim, osys, r64 = make_pair(N, R_n, fl, r, ff=ff, dt=dt, lbs=lbs, cell_reach=reach, lanes_per_bead=lanes,
-                               **({"bal_rounds": rounds} if rounds > 1 else {}))
+                               type_order=typed, **({"bal_rounds": rounds} if rounds > 1 else {}))
     sim.set_time(0, 0.0, 0)
     sim.run_steps(1000)
     sim.synchronize()
@@ -230,9 +237,10 @@ def test_step_grid_bit_exact():
 @pytest.mark.parametrize("reach", [2, 1])
 def test_forces_crowded_lists(reach):
     # Het beads on a cubic lattice of spacing 1 sigma (volume fraction 0.52):
-    # every bead's trimmed neighbour rows hold more slots than the eight-lane
-    # kernel's 128-entry list, so each lane scans its own rows directly (the
-    # crowded path), at reach 2 (1.25 sigma cells) and reach 1 (2.5 sigma).
+    # the inner beads' trimmed neighbour rows hold more slots than the
+    # eight-lane kernel's 128-entry list, so each of their lanes scans its own
+    # rows directly (the crowded path), while the surface beads use the list;
+    # at reach 2 (1.25 sigma cells) and reach 1 (2.5 sigma).
     step(f"forces, crowded lists, reach {reach}")
     k = 12
     N = k ** 3
@@ -256,8 +264,7 @@ def test_forces_crowded_lists(reach):
     occ = {}
     for c in map(tuple, g):
         occ[c] = occ.get(c, 0) + 1
-    lo = (g - cps // 2) * csl
-    shortest = None
+    lengths = []
     for i in range(0, N, 97):
         tot = 0
         for (cx, cy, cz), m in occ.items():
@@ -268,12 +275,16 @@ def test_forces_crowded_lists(reach):
                 gap += d * d
             if gap <= (2.5 + 1e-3) ** 2:
                 tot += m
-        shortest = tot if shortest is None else min(shortest, tot)
-    assert shortest > 128, shortest
+        lengths.append(tot)
+    lengths = np.array(lengths)
+    # the lattice's inner beads overflow the list, its surface beads do not
+    crowded = float((lengths > 128).mean())
+    assert crowded > 0.25 and lengths.min() <= 128, (crowded, lengths.min())
     got = sim.compute_forces(po.FF_LJ).astype(np.float64)
     want = osys.forces(r64, method=0)
     w32 = osys.forces(r64, method=0, prec=32)
     err, err32, scale = np.abs(got - want).max(), np.abs(w32 - want).max(), np.abs(want).max()
-    step(f"  shortest list {shortest}, max|f|={scale:.3g} gpu err={err:.3g} oracle fp32 err={err32:.3g}")
+    step(f"  lists over 128 entries: {crowded:.0%} of the sampled beads; max|f|={scale:.3g} gpu err={err:.3g} "
+         f"oracle fp32 err={err32:.3g}")
     assert err <= 3 * err32 + 1e-6 * scale
     sim.close()

@@ -21,4 +21,9 @@ def test_reduced_sweep_learns_the_radial_density():
          f"vs {err['mean_map_baseline']:.3f}")
     assert err["radial_density_mae"]["surrogate"] < 0.7 * err["radial_density_mae"]["mean_baseline"]
     assert np.isfinite(err["surrogate_top"]) and err["surrogate_validation"] < 1.05 * err["mean_map_baseline"]
-    assert out["vs_plain"] > 1.0
+    # only re-simulated sets whose verified error is within the bar count as resolved
+    assert out["bar"]["contact_map_error"] == 0.05
+    assert 0 <= out["resolved_verified"] <= out["verified"] == 32
+    assert out["value"] == pytest.approx(out["resolved_verified"] * 3600.0 / out["seconds"]["total"])
+    assert (out["resolved_verified"] > 0) == (min(out["top_errors"]) <= 0.05)
+    assert out["scored_per_hr"] > out["value"]

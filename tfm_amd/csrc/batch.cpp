@@ -29,10 +29,12 @@ struct BatchGrid {
   GridGeom g{}; // n_c, cap, nblk of the whole range
   DevBuf<uint32_t> count, bs0, bs1, cstart;
   DevBuf<float4> bucket, spos; // bucket: members' positions, index in the w bits
+  DevBuf<uint32_t> hbs0, hbs1, work; // type-ordered work list (k_force_typed)
   int clean = 0;
   bool valid = false;
   uint32_t *bs(int k) { return k ? bs1.p : bs0.p; }
-  void alloc(size_t n_obj) {
+  uint32_t *hbs(int k) { return k ? hbs1.p : hbs0.p; }
+  void alloc(size_t n_obj, bool typed) {
     count.alloc(g.n_c);
     bs0.alloc(g.nblk);
     bs1.alloc(g.nblk);
@@ -43,14 +45,33 @@ struct BatchGrid {
     hip_check(hipMemset(bs0.p, 0, (size_t)g.nblk * 4));
     hip_check(hipMemset(bs1.p, 0, (size_t)g.nblk * 4));
     hip_check(hipMemset(cstart.p, 0, ((size_t)g.n_c + 1) * 4));
+    if (typed) {
+      hbs0.alloc(g.nblk);
+      hbs1.alloc(g.nblk);
+      work.alloc(n_obj);
+      hip_check(hipMemset(hbs0.p, 0, (size_t)g.nblk * 4));
+      hip_check(hipMemset(hbs1.p, 0, (size_t)g.nblk * 4));
+      hip_check(hipMemset(work.p, 0, std::max<size_t>(n_obj, 1) * 4));
+    }
     clean = 0;
     valid = false;
   }
   void free_() {
-    DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &cstart};
+    DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &cstart, &hbs0, &hbs1, &work};
     for (auto *b : u) b->free_();
     bucket.free_();
     spos.free_();
+  }
+  OrderJob order_job(uint32_t n_het, uint32_t n_slots) {
+    OrderJob j{g, count.p, bs(clean), bs(clean ^ 1), bucket.p, spos.p, nullptr, cstart.p};
+    if (work.p) {
+      j.work = work.p;
+      j.hbs_read = hbs(clean);
+      j.hbs_clear = hbs(clean ^ 1);
+      j.n_het = n_het;
+      j.n_slots = n_slots;
+    }
+    return j;
   }
 };
 
@@ -109,6 +130,13 @@ struct ccp_batch {
   }
 
   void upload() {
+    uint32_t n_het = 0;
+    for (uint32_t i = 0; i < N; ++i) n_het += (hfl[i] & CCP_HET) ? 1u : 0u;
+    if (n_het != (uint32_t)kp.n_het) {
+      // captured graphs hold the kernel parameters by value
+      drop_graphs();
+      set_het_count(kp, n_het);
+    }
     std::vector<float4> h(N);
     for (uint32_t i = 0; i < N; ++i) {
       uint32_t w = i | hbond[i];
@@ -156,9 +184,8 @@ struct ccp_batch {
     if (grid.valid) return;
     for (;;) {
       launch_bin_batch(stream, dsys.p, dslot.p, (int)n_sys, (int)N, r.p, grid.count.p, grid.bs(grid.clean),
-                       grid.bucket.p, flags.p);
-      OrderJob j{grid.g, grid.count.p, grid.bs(grid.clean), grid.bs(grid.clean ^ 1), grid.bucket.p,
-                 grid.spos.p, nullptr, grid.cstart.p};
+                       grid.bucket.p, flags.p, grid.work.p ? grid.hbs(grid.clean) : nullptr);
+      const OrderJob j = grid.order_job((uint32_t)kp.n_het, N);
       launch_order(stream, j, nullptr, flags.p, nullptr);
       hip_check(hipGetLastError());
       grid.clean ^= 1;
@@ -226,6 +253,8 @@ struct ccp_batch {
     q.pstar = pstar.p;
     q.fcount = grid.count.p;
     q.fblocksum = grid.bs(grid.clean);
+    q.fhblocksum = grid.work.p ? grid.hbs(grid.clean) : nullptr;
+    q.work = grid.work.p;
     q.fbucket = grid.bucket.p;
     q.flags = flags.p;
     q.step = dstep.p;
@@ -236,8 +265,7 @@ struct ccp_batch {
     q.sys_slot = dslot.p;
     q.n_sys = (int)n_sys;
     launch_force(stream, mode, stage, k, q);
-    OrderJob j{grid.g, grid.count.p, grid.bs(grid.clean), grid.bs(grid.clean ^ 1), grid.bucket.p,
-               grid.spos.p, nullptr, grid.cstart.p};
+    const OrderJob j = grid.order_job((uint32_t)kp.n_het, N);
     // with the order pass: the next step's normals after the first stage,
     // the bonded terms of q.out0 after both
     const NoiseJob nj = noise_job(1);
@@ -524,7 +552,7 @@ ccp_batch *ccp_batch_create(const char *common_params, const ccp_batch_system *s
     b->dstep.alloc(1);
     b->dover.alloc(n_sys);
     b->obs.alloc((6 + 3 * (size_t)b->rd_bins) * n_sys);
-    b->grid.alloc(b->N);
+    b->grid.alloc(b->N, b->kp.typed != 0);
     hip_check(hipMemcpy(b->dsys.p, b->sys.data(), n_sys * sizeof(SysParams), hipMemcpyHostToDevice));
     hip_check(hipMemcpy(b->dslot.p, b->slot.data(), (n_sys + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
     hip_check(hipMemset(b->flags.p, 0, FLAG_COUNT * sizeof(uint32_t)));

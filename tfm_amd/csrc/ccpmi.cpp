@@ -148,9 +148,20 @@ void ccp_sim::configure(const ParMap &par) {
   // groups of eight beads, then the per-bead work of all of them at once
   rounds = par.get<uint32_t>("bal_rounds", 1u);
   if (rounds != 1 && rounds != 2 && rounds != 4 && rounds != 8) throw Error(CCP_ERR_PARAM, "bal_rounds out of range");
+  // with 8 lanes at reach 2: the force kernel takes the beads from a
+  // type-ordered work list (het beads first, eight lanes each; the others
+  // four lanes each over reach 1), when the short cutoffs of the pair set and
+  // of the relaxation stay within one cell side
+  {
+    const double margin = 1e-3, rs = lj ? std::pow(2.0, 1.0 / 6.0) : RCO;
+    const double reach1 = std::max(rs, (double)RCO) + margin;
+    typed = par.get<uint32_t>("type_order", 1u) != 0 && lanes == 8 && reach == 2 && !split && !integrate_pass &&
+            rounds == 1 && reach1 < csl_f;
+  }
   if (rounds > 1 && integrate_pass) throw Error(CCP_ERR_PARAM, "bal_rounds > 1 excludes integrate_pass");
   graph_steps = par.get<uint32_t>("graph_steps", 64u);
   if (!(1 <= graph_steps && graph_steps <= 4096)) throw Error(CCP_ERR_PARAM, "graph_steps out of range");
+  use_graphs = par.get<uint32_t>("graphs", 1u) != 0;
 
   // chains: ccp's chromosome limits when N = 30386 (chrdat.cuh:27-34),
   // otherwise number_of_chains equal parts or an explicit chain_lengths list
@@ -214,6 +225,8 @@ void ccp_sim::fill_kparams() {
   k.lanes = (int)lanes;
   k.group = group_types ? 1 : 0;
   k.rounds = (int)rounds;
+  k.typed = typed ? 1 : 0;
+  set_het_count(k, 0);
   k.reach = split ? 1 : (int)reach;
   k.R_n = R_n; k.R_o = R_o; k.R_b = R_b;
   k.noc = noc; k.boc = boc; k.nod = nod; k.bod = bod;
@@ -270,7 +283,7 @@ void ccp_sim::allocate() {
   cmap.alloc((size_t)cbins * cbins);
   dstep.alloc(1);
   dcount.alloc(1);
-  gf.alloc(std::max(N, n_l));
+  gf.alloc(std::max(N, n_l), typed);
   if (split) gh.alloc(N);
   hip_check(hipMemset(flags.p, 0, FLAG_COUNT * sizeof(uint32_t)));
   hip_check(hipMemset(cmap.p, 0, (size_t)cbins * cbins * sizeof(uint32_t)));
@@ -292,6 +305,13 @@ uint32_t ccp_sim::wbits(uint32_t i) const {
 }
 
 void ccp_sim::upload_positions() {
+  uint32_t n_het = 0;
+  for (uint32_t i = 0; i < N; ++i) n_het += (hfl[i] & CCP_HET) ? 1u : 0u;
+  if (n_het != (uint32_t)kp.n_het) {
+    // captured graphs hold the kernel parameters by value
+    drop_graphs();
+    set_het_count(kp, n_het);
+  }
   std::vector<float4> h(N);
   for (uint32_t i = 0; i < N; ++i) {
     uint32_t w = wbits(i);
@@ -342,9 +362,11 @@ void ccp_sim::check_flags() {
 // Bin n_obj positions with G's buffers in geometry `geom` and order them
 // into (out_spos, out_cstart[, out_soi]).
 void ccp_sim::build_grid(HostGrid &G, const GridGeom &geom, const float4 *pos, int n_obj, bool het_only,
-                         float4 *out_spos, uint32_t *out_cstart, uint32_t *out_soi) {
-  launch_bin(stream, kp, geom, n_obj, pos, het_only, G.count.p, G.bs(G.clean), G.bucket.p, flags.p);
-  OrderJob j{geom, G.count.p, G.bs(G.clean), G.bs(G.clean ^ 1), G.bucket.p, out_spos, out_soi, out_cstart};
+                         float4 *out_spos, uint32_t *out_cstart, uint32_t *out_soi, bool work_list) {
+  const bool tw = work_list && G.work.p;
+  launch_bin(stream, kp, geom, n_obj, pos, het_only, G.count.p, G.bs(G.clean), G.bucket.p, flags.p,
+             tw ? G.hbs(G.clean) : nullptr);
+  const OrderJob j = G.order_job(geom, tw, (uint32_t)kp.n_het, (uint32_t)n_obj, out_spos, out_cstart, out_soi);
   launch_order(stream, j, nullptr, flags.p, nullptr);
   hip_check(hipGetLastError());
   G.clean ^= 1;
@@ -388,7 +410,7 @@ void ccp_sim::ensure_grids() {
   if (gf.valid && (gh.valid || !split) && lgrid_valid) return;
   for (;;) {
     if (!lgrid_valid) build_grid(gf, lg, lr.p, (int)n_l, false, lspos.p, lstart.p, nullptr);
-    if (!gf.valid) build_grid(gf, gf.g, r.p, (int)N, false, gf.spos.p, gf.cstart.p, nullptr);
+    if (!gf.valid) build_grid(gf, gf.g, r.p, (int)N, false, gf.spos.p, gf.cstart.p, nullptr, typed);
     if (split && !gh.valid) build_grid(gh, gh.g, r.p, (int)N, true, gh.spos.p, gh.cstart.p, nullptr);
     if (!take_overflow()) break;
     grow_capacity();
@@ -421,6 +443,8 @@ void ccp_sim::launch_stage(int mode, int stage, hipEvent_t mid) {
   q.pstar = pstar.p;
   q.fcount = gf.count.p;
   q.fblocksum = gf.bs(gf.clean);
+  q.fhblocksum = typed ? gf.hbs(gf.clean) : nullptr;
+  q.work = typed ? gf.work.p : nullptr;
   q.fbucket = gf.bucket.p;
   q.hcount = gh.count.p;
   q.hblocksum = gh.bs(gh.clean);
@@ -432,8 +456,7 @@ void ccp_sim::launch_stage(int mode, int stage, hipEvent_t mid) {
   q.fbond = bond_pass() ? fbond.p : nullptr;
   launch_force(stream, mode, stage, k, q);
   if (mid) hip_check(hipEventRecord(mid, stream));
-  OrderJob jf{gf.g, gf.count.p, gf.bs(gf.clean), gf.bs(gf.clean ^ 1), gf.bucket.p, gf.spos.p, nullptr,
-              gf.cstart.p};
+  const OrderJob jf = gf.order_job(gf.g, typed, (uint32_t)kp.n_het, N, gf.spos.p, gf.cstart.p, nullptr);
   OrderJob jh{gh.g, gh.count.p, gh.bs(gh.clean), gh.bs(gh.clean ^ 1), gh.bucket.p, gh.spos.p, nullptr,
               gh.cstart.p};
   // With the order pass: after the first stage the counter still holds this
@@ -461,6 +484,8 @@ hipGraphExec_t ccp_sim::step_graph(uint32_t len, int mode) {
     hip_check(hipStreamEndCapture(stream, &g));
     hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     hip_check(hipGraphDestroy(g));
+    // its device-side setup now, not at its first launch
+    hip_check(hipGraphUpload(ge, stream));
     // each step flips the clean buffers twice
     gf.clean = cf;
     gh.clean = ch;
@@ -473,6 +498,11 @@ hipGraphExec_t ccp_sim::step_graph(uint32_t len, int mode) {
 // rest as graphs of the powers of two in its binary expansion.
 void ccp_sim::run_chunk(uint64_t n, int mode) {
   if (n == 0) return;
+  if (!use_graphs) {
+    for (uint64_t s = 0; s < n; ++s) launch_step(mode);
+    hip_check(hipGetLastError());
+    return;
+  }
   const uint64_t G = graph_steps;
   const int cf = gf.clean, ch = gh.clean;
   const bool vh = gh.valid;
@@ -489,6 +519,7 @@ void ccp_sim::run_chunk(uint64_t n, int mode) {
 }
 
 void ccp_sim::capture_graphs(int mode) {
+  if (!use_graphs) return;
   for (uint32_t len = 1; len < graph_steps; len <<= 1) step_graph(len, mode);
   step_graph(graph_steps, mode);
 }
@@ -1579,7 +1610,7 @@ int ccp_analysis_end_simulation(ccp_analysis *a, const char *path) {
     f << "         avg   sqrt(var)         sem   f_n_b     i_t ter\n";
     for (uint32_t k = 0; k < 4; ++k) {
       const SeriesStat &t = a->sv[k].back();
-      f << stat3(t) << cnfs(t.f_n_b, 8, ' ') << cnfs(t.i_t, 8, ' ') << (t.thm ? " yes" : "  no") << "\n";
+      f << stat3(t) << cnfs(t.blocks, 8, ' ') << cnfs(t.cut, 8, ' ') << (t.settled ? " yes" : "  no") << "\n";
     }
     f << "\n\n";
     for (uint32_t ty = 0; ty < 3; ++ty) {
@@ -1686,9 +1717,9 @@ int ccp_series_stats(const float *v, size_t n, double out[6]) {
   out[0] = s.avg;
   out[1] = s.var;
   out[2] = s.sem;
-  out[3] = s.f_n_b;
-  out[4] = s.i_t;
-  out[5] = s.thm ? 1.0 : 0.0;
+  out[3] = s.blocks;
+  out[4] = s.cut;
+  out[5] = s.settled ? 1.0 : 0.0;
   return CCP_OK;
 }
 

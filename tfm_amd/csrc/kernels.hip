@@ -176,8 +176,10 @@ __device__ __forceinline__ int cell_coord(float x, const GridGeom &g, bool &outs
 }
 
 // The bucket entry is the member's position with its index in the w bits.
+// hblocksum (optional) counts the het members per scan block, for the order
+// pass's type-ordered work list.
 __device__ __forceinline__ void bin_into(const GridGeom &g, uint32_t *count, uint32_t *blocksum, float4 *bucket,
-                                         uint32_t *flags, float4 v) {
+                                         uint32_t *flags, float4 v, uint32_t *hblocksum = nullptr) {
   bool outside = false;
   const int gx = cell_coord(v.x, g, outside), gy = cell_coord(v.y, g, outside), gz = cell_coord(v.z, g, outside);
   const uint32_t key = (uint32_t)(g.coff + (gz * g.cps + gy) * g.cps + gx);
@@ -185,6 +187,7 @@ __device__ __forceinline__ void bin_into(const GridGeom &g, uint32_t *count, uin
   const uint32_t slot = atomicAdd(count + key, 1u);
   // the block totals and the flag need no slot: done while it returns
   agg_add_noret(blocksum, key / SCAN_BLOCK);
+  if (hblocksum && (wbits(v) & W_HET)) agg_add_noret(hblocksum, key / SCAN_BLOCK);
   if (outside) atomicAdd(flags + FLAG_OUTSIDE, 1u);
   if (slot < (uint32_t)g.cap) bucket[(size_t)key * g.cap + slot] = v;
 }
@@ -193,7 +196,7 @@ __device__ __forceinline__ void bin_into(const GridGeom &g, uint32_t *count, uin
 // next het grid (the het-grid atomics run on the het lanes only).
 __device__ __forceinline__ void bin_bead(const KParams &p, const KPtrs &q, uint32_t w, float x, float y, float z) {
   const float4 v = make_float4(x, y, z, __uint_as_float(w)); // w carries the bead index
-  bin_into(p.gf, q.fcount, q.fblocksum, q.fbucket, q.flags, v);
+  bin_into(p.gf, q.fcount, q.fblocksum, q.fbucket, q.flags, v, q.fhblocksum);
   if (p.gh.n_c > 0 && (w & W_HET)) bin_into(p.gh, q.hcount, q.hblocksum, q.hbucket, q.flags, v);
   if (!(isfinite(x) && isfinite(y) && isfinite(z))) atomicAdd(q.flags + FLAG_NONFINITE, 1u);
 }
@@ -526,11 +529,12 @@ __device__ __forceinline__ void pair_term(const KParams &p, const PairCtx &c, fl
 // L2. Renumber them so XCD x runs one contiguous run of tiles: its beads,
 // and the neighbour rows they read, then stay in that XCD's L2.
 constexpr int N_XCD = 8;
-__device__ __forceinline__ int xcd_block() {
-  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+// block b of a run of nb blocks whose first block sits on XCD 0
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
   const int x = b % N_XCD, k = b / N_XCD, q = nb / N_XCD, rem = nb % N_XCD;
   return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + k;
 }
+__device__ __forceinline__ int xcd_block() { return xcd_remap((int)blockIdx.x, (int)gridDim.x); }
 
 // The 2R+1 neighbour rows (y-R..y+R, x-R..x+R) of plane z + dz around the
 // cell holding (x, y, z); rows outside the grid get an empty range. Reach R
@@ -900,16 +904,20 @@ constexpr uint64_t ring2_word(int m) {
 constexpr uint64_t RING2_WORDS[3] = {ring2_word(0), ring2_word(1), ring2_word(2)};
 
 // Pair (and LBS) forces of the bead `me` over its trimmed candidate list,
-// spread over the bead's eight lanes (sub = 0..7) and added to f; on return
-// every lane of the group holds the bead's total. `list` is the group's LDS
-// list, `self` the bead's own cell-order slot; the caller's lanes of one
-// wavefront run this together.
-template <int MODE, bool BATCH, int R>
+// spread over the bead's G lanes (sub = 0..G-1; G = 8, or 4 for beads whose
+// cutoff stays within one cell at reach 1) and added to f; on return every
+// lane of the group holds the bead's total. `list` is the group's LDS list of
+// LCAP entries, `self` the bead's own cell-order slot; the caller's lanes of
+// one wavefront run this together.
+template <int MODE, bool BATCH, int R, int G = C_LANES, int LCAP = C_LIST>
 __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, uint32_t *list, bool active,
                                              uint32_t self, int sub, int lane, float4 me, float &fx, float &fy,
                                              float &fz) {
+  static_assert(G == 8 || G == 4, "eight or four lanes per bead");
+  static_assert(G >= 2 * R + 1, "the group's lanes hold the gaps of offsets -R..R");
+  static_assert(R == 1 || G == 8, "reach 2 takes the ring order of eight lanes");
   // NROW: the rows other than the centre one, RPL of them per lane
-  constexpr int W = 2 * R + 1, NROW = W * W - 1, RPL = (NROW + C_LANES - 1) / C_LANES;
+  constexpr int W = 2 * R + 1, NROW = W * W - 1, RPL = (NROW + G - 1) / G;
   const uint32_t w = wbits(me);
   const float rx = me.x, ry = me.y, rz = me.z;
   bool outside = false;
@@ -921,7 +929,7 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   const AxisGaps<R> GX = axis_gaps<R>(rx - (float)(cx - p.gf.half) * csl, csl, cx, cps);
   // y and z: lane sub of the bead's group holds the squared gaps to the
   // cells d = sub - R steps away; each row takes its two from those lanes
-  const int d = sub - R, gb4 = (lane & ~(C_LANES - 1)) << 2;
+  const int d = sub - R, gb4 = (lane & ~(G - 1)) << 2;
   const float gyl = axis_gap2(ry - (float)(cy - p.gf.half) * csl, csl, cy, d, cps),
               gzl = axis_gap2(rz - (float)(cz - p.gf.half) * csl, csl, cz, d, cps);
   // cell (cx, cy + iy - R, cz + iz - R) is base0 + (iz cps + iy) cps; the
@@ -967,7 +975,7 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     // bead within sqrt(rc2), empty when the row's y-z slab is farther (or
     // outside the grid), else the run around cx whose x gaps fit in the rest,
     // rc2 - gap_y^2 - gap_z^2. Branch-free: an empty row selects [0, 0).
-    const int r = sub + m * C_LANES;
+    const int r = sub + m * G;
     // row r of the W x W rows with the centre (index W * W / 2) left out
     const int rr = r < W * W / 2 ? r : r + 1;
     int iy = rr % W, iz = rr / W;
@@ -1031,20 +1039,20 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     incl += sub >= 1 ? v1 : 0u;
     const uint32_t v2 = dpp_u<DPP_SHR2>(incl);
     incl += sub >= 2 ? v2 : 0u;
-    incl += dpp_u<DPP_SHR4, 0xA>(incl);
+    if constexpr (G == 8) incl += dpp_u<DPP_SHR4, 0xA>(incl);
   }
   uint32_t total = tot;
   total += dpp_u<DPP_XOR1>(total);
   total += dpp_u<DPP_XOR2>(total);
-  total += dpp_u<DPP_MIRROR8>(total);
+  if constexpr (G == 8) total += dpp_u<DPP_MIRROR8>(total);
   // the centre row's entries come first
   total += clen;
   uint32_t off = clen + incl - tot;
-  const bool listed = total <= (uint32_t)C_LIST;
+  const bool listed = total <= (uint32_t)LCAP;
   // the list holds byte offsets (slot x 16) into the ordered positions, so a
   // gather needs no address arithmetic
   if (listed) {
-    for (uint32_t k = sub; k < clen; k += C_LANES) list[k] = (clo + k) << 4;
+    for (uint32_t k = sub; k < clen; k += G) list[k] = (clo + k) << 4;
 #pragma unroll
     for (int m = 0; m < RPL; ++m) {
       uint32_t b = lo[m] << 4;
@@ -1067,31 +1075,31 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     // same entries in the same order whatever the width.
     uint32_t base = 0;
     // long lists (het beads): eight gathers in flight
-    for (; base + 8 * C_LANES <= total; base += 8 * C_LANES) {
+    for (; base + 8 * G <= total; base += 8 * G) {
       const uint32_t n = base + sub;
       float4 e[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) e[k] = at_byte(q.fspos, list[n + k * C_LANES]);
+      for (int k = 0; k < 8; ++k) e[k] = at_byte(q.fspos, list[n + k * G]);
 #pragma unroll
       for (int k = 0; k < 8; ++k) pair_term<MODE>(p, pc, e[k], fx, fy, fz);
     }
-    for (; base + 4 * C_LANES <= total; base += 4 * C_LANES) {
+    for (; base + 4 * G <= total; base += 4 * G) {
       const uint32_t n = base + sub;
-      const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, list[n + C_LANES]),
-                   c = at_byte(q.fspos, list[n + 2 * C_LANES]), d = at_byte(q.fspos, list[n + 3 * C_LANES]);
+      const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, list[n + G]),
+                   c = at_byte(q.fspos, list[n + 2 * G]), d = at_byte(q.fspos, list[n + 3 * G]);
       pair_term<MODE>(p, pc, a, fx, fy, fz);
       pair_term<MODE>(p, pc, b, fx, fy, fz);
       pair_term<MODE>(p, pc, c, fx, fy, fz);
       pair_term<MODE>(p, pc, d, fx, fy, fz);
     }
     const uint32_t rem = total - base, n = base + sub, s4 = self << 4;
-    const bool wide = __ballot(rem > 3 * C_LANES) != 0ull;
+    const bool wide = __ballot(rem > 3 * G) != 0ull;
     if (n < total) {
-      const uint32_t jb = n + C_LANES < total ? list[n + C_LANES] : s4;
-      const uint32_t jc = n + 2 * C_LANES < total ? list[n + 2 * C_LANES] : s4;
+      const uint32_t jb = n + G < total ? list[n + G] : s4;
+      const uint32_t jc = n + 2 * G < total ? list[n + 2 * G] : s4;
       const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, jb), c = at_byte(q.fspos, jc);
       float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (wide) d = at_byte(q.fspos, n + 3 * C_LANES < total ? list[n + 3 * C_LANES] : s4);
+      if (wide) d = at_byte(q.fspos, n + 3 * G < total ? list[n + 3 * G] : s4);
       pair_term<MODE>(p, pc, a, fx, fy, fz);
       pair_term<MODE>(p, pc, b, fx, fy, fz);
       pair_term<MODE>(p, pc, c, fx, fy, fz);
@@ -1100,7 +1108,7 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   } else {
     // crowded neighbourhood: each lane scans its own rows, and every eighth
     // entry of the centre row
-    for (uint32_t k = sub; k < clen; k += C_LANES) pair_term<MODE>(p, pc, q.fspos[clo + k], fx, fy, fz);
+    for (uint32_t k = sub; k < clen; k += G) pair_term<MODE>(p, pc, q.fspos[clo + k], fx, fy, fz);
 #pragma unroll
     for (int m = 0; m < RPL; ++m)
       for (uint32_t k = 0; k < len[m]; ++k) pair_term<MODE>(p, pc, q.fspos[lo[m] + k], fx, fy, fz);
@@ -1110,7 +1118,9 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   // every lane adds the same two values each time, so all end equal
   fx += dpp_f<DPP_XOR1>(fx); fy += dpp_f<DPP_XOR1>(fy); fz += dpp_f<DPP_XOR1>(fz);
   fx += dpp_f<DPP_XOR2>(fx); fy += dpp_f<DPP_XOR2>(fy); fz += dpp_f<DPP_XOR2>(fz);
-  fx += dpp_f<DPP_MIRROR8>(fx); fy += dpp_f<DPP_MIRROR8>(fy); fz += dpp_f<DPP_MIRROR8>(fz);
+  if constexpr (G == 8) {
+    fx += dpp_f<DPP_MIRROR8>(fx); fy += dpp_f<DPP_MIRROR8>(fy); fz += dpp_f<DPP_MIRROR8>(fz);
+  }
   // every lane's list reads are done before the group's next list is written
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1231,6 +1241,70 @@ __global__ __launch_bounds__(BAL_BLOCK) void k_force_bal(KParams p, KPtrs q) {
   }
 }
 
+// The balanced kernel over a type-ordered work list (q.work: the cell-order
+// slots of the het beads, then those of the others, each part in cell order,
+// written by the order pass). Every wavefront holds beads of one type:
+// - the first p.het_blocks workgroups take the het beads, eight lanes per
+//   bead over the 5 x 5 x 5 half-size cells (their cutoff spans two cells);
+// - the rest take the others, four lanes per bead over 3 x 3 x 3 of the same
+//   cells (their cutoff is shorter than one cell side, checked by the host),
+//   32 beads per workgroup.
+// Each part's workgroups are renumbered per XCD on their own, so every XCD
+// gets its share of both. Results are those of k_force_bal: the same pair
+// set, summed in another order.
+constexpr int T_LIST_SHORT = 64; // list entries per bead of the four-lane part
+template <int MODE, int STAGE, bool BATCH>
+__global__ __launch_bounds__(BAL_BLOCK) void k_force_typed(KParams p, KPtrs q) {
+  __shared__ uint32_t s_list[C_BEADS_PER_BLOCK * C_LIST];
+  static_assert(C_BEADS_PER_BLOCK * C_LIST == 2 * C_BEADS_PER_BLOCK * T_LIST_SHORT, "both parts share the list");
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const int b = (int)blockIdx.x;
+  const bool het_part = b < p.het_blocks;
+  int t, sub, grp;
+  uint32_t end;
+  if (het_part) {
+    sub = lane % C_LANES;
+    grp = wave * (64 / C_LANES) + lane / C_LANES;
+    t = xcd_remap(b, p.het_blocks) * C_BEADS_PER_BLOCK + grp;
+    end = (uint32_t)p.n_het;
+  } else {
+    sub = lane % 4;
+    grp = wave * 16 + lane / 4;
+    t = p.n_het + xcd_remap(b - p.het_blocks, (int)gridDim.x - p.het_blocks) * 2 * C_BEADS_PER_BLOCK + grp;
+    end = (uint32_t)p.N;
+  }
+  const bool in = (uint32_t)t < end;
+  // the work list holds slots below N; min() keeps a stale entry (left by an
+  // overflowing cell, which is reported) inside the arrays
+  const uint32_t slot = in ? min(q.work[t], (uint32_t)p.N - 1u) : 0u;
+  const float4 me = in ? q.fspos[slot] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const uint32_t w = wbits(me);
+  const uint32_t i = w & ID_MASK;
+  const bool active = in && i < (uint32_t)p.N; // a sentinel slot holds no bead
+  if constexpr (BATCH) load_system(p, q, active ? slot : 0u);
+  const bool lead = sub == 0;
+  float4 inp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (active && lead) inp = stage_input<STAGE>(p, q, i);
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (active && lead) {
+    if (q.fbond) {
+      const float4 fb = q.fbond[i];
+      fx = fb.x; fy = fb.y; fz = fb.z;
+    } else {
+      bonded_force(p, q.pos, i, w, me.x, me.y, me.z, fx, fy, fz);
+      wall_force<MODE>(p, me.x, me.y, me.z, w, fx, fy, fz);
+    }
+  }
+  if (het_part)
+    bal_pair_sum<MODE, BATCH, 2, C_LANES, C_LIST>(p, q, s_list + grp * C_LIST, active, slot, sub, lane, me, fx, fy,
+                                                   fz);
+  else
+    bal_pair_sum<MODE, BATCH, 1, 4, T_LIST_SHORT>(p, q, s_list + grp * T_LIST_SHORT, active, slot, sub, lane, me,
+                                                   fx, fy, fz);
+  if (!active || !lead) return;
+  stage_update<STAGE>(p, q, i, w, me, fx, fy, fz, inp);
+}
+
 // Bonds, wall and the Heun stage of every bead, one lane per bead in index
 // order, on top of the pair forces k_force_bal<..., false> left in q.fpair.
 template <int MODE, int STAGE, bool BATCH>
@@ -1251,24 +1325,25 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_integrate(KParams p, KPtrs q) {
 // Binning of an index-ordered array (initial builds, LBS grid, export grid).
 __global__ __launch_bounds__(BEAD_BLOCK) void k_bin(GridGeom g, int n_obj, const float4 *pos, int het_only,
                                                     uint32_t *count, uint32_t *blocksum, float4 *bucket,
-                                                    uint32_t *flags) {
+                                                    uint32_t *flags, uint32_t *hblocksum) {
   const int i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
   if (i >= n_obj) return;
   const float4 v = pos[i];
   if (het_only && !(wbits(v) & W_HET)) return;
-  bin_into(g, count, blocksum, bucket, flags, v);
+  bin_into(g, count, blocksum, bucket, flags, v, hblocksum);
 }
 
 // Binning of the index-ordered beads of a batch into their systems' grids
 // (bead i belongs to the system whose slot range holds i).
 __global__ __launch_bounds__(BEAD_BLOCK) void k_bin_batch(const SysParams *sys, const uint32_t *sys_slot, int n_sys,
                                                           int n, const float4 *pos, uint32_t *count,
-                                                          uint32_t *blocksum, float4 *bucket, uint32_t *flags) {
+                                                          uint32_t *blocksum, float4 *bucket, uint32_t *flags,
+                                                          uint32_t *hblocksum) {
   const int i = blockIdx.x * BEAD_BLOCK + threadIdx.x;
   if (i >= n) return;
   const float4 v = pos[i];
   const SysParams &S = sys[system_of(sys_slot, n_sys, (uint32_t)i)];
-  bin_into(S.g, count, blocksum, bucket, flags, v);
+  bin_into(S.g, count, blocksum, bucket, flags, v, hblocksum);
 }
 
 // Per-cell scan + deterministic order of one or two grids (replaces ccp's
@@ -1282,11 +1357,13 @@ constexpr int ORDER_LDS = 1024;
 __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, uint32_t *flags,
                                                       uint64_t *step_inc, NoiseJob nj, KParams bp, BondJob bj) {
   __shared__ uint32_t red[SCAN_BLOCK / 64], wsum[SCAN_BLOCK / 64], msum[SCAN_BLOCK / 64];
+  __shared__ uint32_t hred[SCAN_BLOCK / 64], hsum[SCAN_BLOCK / 64];
   __shared__ uint32_t s_id[ORDER_LDS];
   __shared__ float4 s_pos[ORDER_LDS];
   __shared__ uint8_t s_cell[ORDER_LDS];
   __shared__ uint32_t s_loff[SCAN_BLOCK + 1];
   __shared__ uint32_t s_start[SCAN_BLOCK];
+  __shared__ uint32_t s_hoff[SCAN_BLOCK];
   const int bx = xcd_block();
   const bool second = bx >= ja.g.nblk;
   const OrderJob &J = second ? jb : ja;
@@ -1294,16 +1371,20 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
   const int c = b * SCAN_BLOCK + tid;
   const int cap = J.g.cap, n_c = J.g.n_c;
+  // type-ordered work list of this grid (workgroup-uniform)
+  const bool typed = J.work != nullptr;
   // The ordering's first loads go out before the per-bead terms below, so
   // that their latency overlaps that work: this cell's count and the totals
-  // of the workgroups before this one (four per lane at once).
+  // of the workgroups before this one (four per lane at once; with the work
+  // list also their het totals).
   constexpr int PRE = 4;
   const uint32_t cnt = (c < n_c) ? J.count[c] : 0u;
-  uint32_t pv[PRE];
+  uint32_t pv[PRE], hv[PRE];
 #pragma unroll
   for (int u = 0; u < PRE; ++u) {
     const int k = tid + u * SCAN_BLOCK;
     pv[u] = k < b ? J.bs_read[k] : 0u;
+    hv[u] = typed && k < b ? J.hbs_read[k] : 0u;
   }
   // Per-bead terms of the next force pass at one lane per bead, where the
   // force kernels would run them on one lane of eight: the next step's
@@ -1324,16 +1405,23 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
     }
   }
   uint32_t part = (pv[0] + pv[1]) + (pv[2] + pv[3]);
+  uint32_t hpart = (hv[0] + hv[1]) + (hv[2] + hv[3]);
   for (int k0 = tid + PRE * SCAN_BLOCK; k0 < b; k0 += PRE * SCAN_BLOCK) {
 #pragma unroll
     for (int u = 0; u < PRE; ++u) {
       const int k = k0 + u * SCAN_BLOCK;
       pv[u] = k < b ? J.bs_read[k] : 0u;
+      hv[u] = typed && k < b ? J.hbs_read[k] : 0u;
     }
     part += (pv[0] + pv[1]) + (pv[2] + pv[3]);
+    hpart += (hv[0] + hv[1]) + (hv[2] + hv[3]);
   }
   part = wave_sum(part);
   if (lane == 0) red[wid] = part;
+  if (typed) {
+    hpart = wave_sum(hpart);
+    if (lane == 0) hred[wid] = hpart;
+  }
   uint32_t m = cnt;
   if (m > (uint32_t)cap) {
     atomicAdd(flags + FLAG_OVERFLOW, 1u);
@@ -1353,8 +1441,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
     wsum[wid] = incl;
     msum[wid] = mincl;
   }
+  // het members of this cell (the work list places them before the others)
+  uint32_t hc = 0, hincl = 0;
+  if (typed) {
+#pragma unroll
+    for (uint32_t k = 0; k < EARLY; ++k)
+      if (k < m) hc += (wbits(ev[k]) & W_HET) ? 1u : 0u;
+    for (uint32_t k = EARLY; k < m; ++k) hc += (wbits(bk[k]) & W_HET) ? 1u : 0u;
+    hincl = wave_incl_scan(hc);
+    if (lane == 63) hsum[wid] = hincl;
+  }
   __syncthreads();
-  uint32_t prefix = 0, woff = 0, moff = 0, mtotal = 0;
+  uint32_t prefix = 0, woff = 0, moff = 0, mtotal = 0, hprefix = 0, hoff = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_BLOCK / 64; ++k) {
     prefix += red[k];
@@ -1363,6 +1461,16 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
       woff += wsum[k];
       moff += msum[k];
     }
+  }
+  if (typed) {
+#pragma unroll
+    for (int k = 0; k < SCAN_BLOCK / 64; ++k) {
+      hprefix += hred[k];
+      if (k < wid) hoff += hsum[k];
+    }
+    // het slots before this cell's: those of the blocks before and of the
+    // cells before it in this block
+    hoff += hprefix + hincl - hc;
   }
   const uint32_t start = prefix + woff + incl - cnt;
   moff += mincl - m;
@@ -1373,11 +1481,21 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
   if (c == n_c - 1) J.cstart[n_c] = start + cnt;
   if (tid == 0) {
     J.bs_clear[b] = 0u;
+    // the grid's het totals are cleared by every ordering of its buffers
+    // (those with no work list too), so that the next binning finds them zero
+    if (J.hbs_clear) J.hbs_clear[b] = 0u;
     if (b == 0 && !second && step_inc) *step_inc += 1ull;
   }
+  // work-list entry of the member at slot dst, with hbefore het members in
+  // the slots before it: het members in slot order, then the others
+  auto put_work = [&](uint32_t dst, uint32_t hbefore, bool het) {
+    const uint32_t k = het ? hbefore : J.n_het + (dst - hbefore);
+    if (k < J.n_slots) J.work[k] = dst;
+  };
   if (mtotal <= (uint32_t)ORDER_LDS) {
     s_loff[tid] = moff;
     s_start[tid] = start;
+    s_hoff[tid] = hoff;
     if (tid == SCAN_BLOCK - 1) s_loff[SCAN_BLOCK] = moff + m;
     auto put = [&](uint32_t k, float4 v) {
       s_pos[moff + k] = v;
@@ -1392,21 +1510,37 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
     for (uint32_t e = tid; e < mtotal; e += SCAN_BLOCK) {
       const uint32_t l = s_cell[e], id = s_id[e];
       const uint32_t b0 = s_loff[l], b1 = s_loff[l + 1];
-      uint32_t rank = 0;
-      for (uint32_t s = b0; s < b1; ++s) rank += (s_id[s] < id) ? 1u : 0u;
+      uint32_t rank = 0, hr = 0;
+      if (typed) {
+        for (uint32_t s2 = b0; s2 < b1; ++s2) {
+          const bool before = s_id[s2] < id;
+          rank += before ? 1u : 0u;
+          hr += (before && (wbits(s_pos[s2]) & W_HET)) ? 1u : 0u;
+        }
+      } else {
+        for (uint32_t s2 = b0; s2 < b1; ++s2) rank += (s_id[s2] < id) ? 1u : 0u;
+      }
       const uint32_t dst = s_start[l] + rank;
-      J.spos[dst] = s_pos[e];
+      const float4 v = s_pos[e];
+      J.spos[dst] = v;
       if (J.soi) J.soi[dst] = id;
+      if (typed) put_work(dst, s_hoff[l] + hr, (wbits(v) & W_HET) != 0u);
     }
   } else {
     // dense block (raw initial walks): rank each cell's entries in place
     for (uint32_t k = 0; k < m; ++k) {
       const float4 v = bk[k];
       const uint32_t e = __float_as_uint(v.w) & ID_MASK;
-      uint32_t rank = 0;
-      for (uint32_t s = 0; s < m; ++s) rank += (__float_as_uint(bk[s].w) & ID_MASK) < e ? 1u : 0u;
+      uint32_t rank = 0, hr = 0;
+      for (uint32_t s2 = 0; s2 < m; ++s2) {
+        const float4 o = bk[s2];
+        const bool before = (__float_as_uint(o.w) & ID_MASK) < e;
+        rank += before ? 1u : 0u;
+        hr += (before && (wbits(o) & W_HET)) ? 1u : 0u;
+      }
       J.spos[start + rank] = v;
       if (J.soi) J.soi[start + rank] = e;
+      if (typed) put_work(start + rank, hoff + hr, (wbits(v) & W_HET) != 0u);
     }
   }
   // A cell with more members than its bucket holds (flagged above; the host
@@ -1416,6 +1550,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_order(OrderJob ja, OrderJob jb, 
   for (uint32_t k = m; k < cnt; ++k) {
     J.spos[start + k] = make_float4(1e18f, 1e18f, 1e18f, __uint_as_float(ID_MASK));
     if (J.soi) J.soi[start + k] = ID_MASK;
+    if (typed) put_work(start + k, hoff + hc, false);
   }
 }
 
@@ -1688,11 +1823,11 @@ inline unsigned blocks_for(int n, int bs) { return (unsigned)((n + bs - 1) / bs)
 } // namespace
 
 void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, const float4 *pos, bool het_only,
-                uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags) {
+                uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags, uint32_t *hblocksum) {
   (void)p;
   if (n_obj <= 0) return;
   k_bin<<<blocks_for(n_obj, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(g, n_obj, pos, het_only ? 1 : 0, count, blocksum,
-                                                            bucket, flags);
+                                                            bucket, flags, hblocksum);
 }
 
 void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc,
@@ -1730,6 +1865,16 @@ static void force_variant(hipStream_t s, int stage, const KParams &p, const KPtr
 
 template <int MODE, bool BATCH, int R>
 static void force_balanced(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
+  if constexpr (R == 2) {
+    if (p.typed && q.work) {
+      // het beads in het_blocks workgroups of 16, the others in workgroups of 32
+      const unsigned gt = (unsigned)p.het_blocks + blocks_for(p.N - p.n_het, 2 * C_BEADS_PER_BLOCK);
+      if (stage == 0) k_force_typed<MODE, 0, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q);
+      else if (stage == 1) k_force_typed<MODE, 1, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q);
+      else k_force_typed<MODE, 2, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q);
+      return;
+    }
+  }
   const unsigned g = blocks_for(p.N, C_BEADS_PER_BLOCK);
   if (q.fpair) {
     // pair sum with eight lanes per bead, then bonds, wall and the stage with one
@@ -1853,10 +1998,11 @@ void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, f
 }
 
 void launch_bin_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n,
-                      const float4 *pos, uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags) {
+                      const float4 *pos, uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags,
+                      uint32_t *hblocksum) {
   if (n <= 0) return;
   k_bin_batch<<<blocks_for(n, BEAD_BLOCK), BEAD_BLOCK, 0, s>>>(sys, sys_slot, n_sys, n, pos, count, blocksum, bucket,
-                                                              flags);
+                                                              flags, hblocksum);
 }
 
 void launch_close_sums(hipStream_t s, double *sum, double *m1, double *m2, uint32_t n, double n_dp) {

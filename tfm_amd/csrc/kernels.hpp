@@ -72,6 +72,10 @@ struct KParams {
   int lanes;   // lanes per bead of the force kernel: 2 reach + 1 (one per z-plane), 1, or 8 (balanced list)
   int group;   // 8 lanes: within a workgroup, beads with the long (het) cutoff take the first wavefronts
   int rounds;  // 8 lanes: beads per lane group of a wavefront (1, 2, 4 or 8), eight per round of the pair phase
+  // 8 lanes at reach 2 over a type-ordered work list (k_force_typed): the
+  // het beads (n_het of them) in the first het_blocks workgroups, eight lanes
+  // each; the others four lanes each over reach 1 of the same cells
+  int typed, n_het, het_blocks;
   int frozen;  // batches: the bead's system is frozen (set per bead from its SysParams)
   // nucleus geometry (cngeom, chrdat.cuh:48-69)
   float R_n, R_o, R_b, noc, boc, nod, bod;
@@ -110,6 +114,7 @@ struct KPtrs {
   float4 *out0;            // stage 1: er, stage 2: new r, stage 0: force
   float4 *pstar;           // r + f dt/2 + noise (written by stage 1, read by stage 2)
   uint32_t *fcount, *fblocksum; // binning of the output into the next fine grid
+  uint32_t *fhblocksum;    // ... its per-block het totals (type-ordered work list; nullptr: none)
   float4 *fbucket;
   uint32_t *hcount, *hblocksum; // ... and into the next het grid
   float4 *hbucket;
@@ -118,6 +123,7 @@ struct KPtrs {
   float4 *fpair;           // pair forces by bead index (balanced kernel with a separate integration pass)
   const float4 *noise;     // unit normal triples of this step by bead index (first Heun stage)
   const float4 *fbond;     // bonds + bending + wall per bead by bead index (balanced kernel; nullptr: inline)
+  const uint32_t *work;    // type-ordered work list of the fine grid (k_force_typed)
   // batches only (sys == nullptr for a single system)
   const SysParams *sys;    // n_sys systems
   const uint32_t *sys_slot; // n_sys + 1 first slots
@@ -134,6 +140,13 @@ struct OrderJob {
   float4 *spos;
   uint32_t *soi;
   uint32_t *cstart;
+  // type-ordered work list (nullptr: none): the slots of het members first,
+  // then the others, each part in slot order; per-block het totals of the
+  // binning (read, and the other buffer cleared); n_het het beads of n_slots
+  uint32_t *work = nullptr;
+  const uint32_t *hbs_read = nullptr;
+  uint32_t *hbs_clear = nullptr;
+  uint32_t n_het = 0, n_slots = 0;
 };
 
 // Unit normal triples of one step for every bead, by bead index: Philox
@@ -163,9 +176,17 @@ struct BondJob {
   int n_sys;
 };
 
+// Het count of the type-ordered work list and the workgroups of its het part
+// (16 beads each, rounded to a multiple of the 8 XCDs so that the other part
+// starts on XCD 0).
+inline void set_het_count(KParams &k, uint32_t n_het) {
+  k.n_het = (int)n_het;
+  k.het_blocks = (int)(((n_het + 15u) / 16u + 7u) / 8u * 8u);
+}
+
 // Launchers (kernels.hip). All are asynchronous on `s`.
 void launch_bin(hipStream_t s, const KParams &p, const GridGeom &g, int n_obj, const float4 *pos, bool het_only,
-                uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags);
+                uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags, uint32_t *hblocksum = nullptr);
 // The order pass also fills the next force pass's per-bead terms when given:
 // normals (noise) and bonded + wall forces (bp, bond).
 void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t *flags, uint64_t *step_inc,
@@ -187,7 +208,8 @@ void launch_sd_cp(hipStream_t s, const float4 *pos, uint32_t N, uint32_t lsdcp, 
                   float *sd, float *cp, double *sum_sd, double *sum_cp);
 void launch_ccp_cm(hipStream_t s, uint32_t N, uint32_t lcm, const float4 *pos, float *cm, double *sum);
 void launch_bin_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n,
-                      const float4 *pos, uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags);
+                      const float4 *pos, uint32_t *count, uint32_t *blocksum, float4 *bucket, uint32_t *flags,
+                      uint32_t *hblocksum = nullptr);
 // every system of a batch in one launch: contact maps (maps + s bins^2) and
 // the all-bead radial density (out + s n_b, FP64 sums)
 void launch_contacts_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n, float rc,

@@ -1,5 +1,5 @@
 // gfx950 kernels of the contact-map surrogate: a strided, batched BF16 GEMM
-// on the matrix cores (v_mfma_f32_32x32x8 bf16, FP32 accumulation) and the
+// on the matrix cores (v_mfma_f32_32x32x16 bf16, FP32 accumulation) and the
 // element-wise pieces of a 1-D convolutional encoder with a pairwise
 // outer-product head (im2col / col2im, bias + GELU and its derivative, the
 // symmetric pair map, pooling, losses, AdamW).
@@ -17,7 +17,7 @@
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -29,47 +29,71 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-constexpr int TM = 64, TN = 64, TK = 16, GEMM_THREADS = 256;
+// Block tile 64 x 64 (four wavefronts of 32 x 32), K tile 32: two
+// v_mfma_f32_32x32x16_bf16 per wavefront per K tile.
+constexpr int TM = 64, TN = 64, TK = 32, GEMM_THREADS = 256;
+// LDS rows: 32 K values plus 8 of padding (80 bytes), so the 16-byte reads
+// of the 32 rows one MFMA operand takes spread over the banks
+constexpr int TKP = TK + 8;
+
+// Eight elements along K of one row (A) or column (B) of the tile, for the
+// staging thread: one 16-byte load when they are contiguous and aligned in
+// memory, else eight loads (zero outside the matrix).
+__device__ __forceinline__ uint4 load8(const uint16_t *__restrict__ P, long base, long sk, int row_ok, int k,
+                                       int K) {
+  if (row_ok && sk == 1 && k + 8 <= K && ((uintptr_t)(P + base + k) & 15u) == 0)
+    return *(const uint4 *)(P + base + k);
+  uint16_t v[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = (row_ok && k + j < K) ? P[base + (long)(k + j) * sk] : (uint16_t)0;
+  uint4 r;
+  r.x = v[0] | (uint32_t)v[1] << 16;
+  r.y = v[2] | (uint32_t)v[3] << 16;
+  r.z = v[4] | (uint32_t)v[5] << 16;
+  r.w = v[6] | (uint32_t)v[7] << 16;
+  return r;
+}
 
 __global__ __launch_bounds__(GEMM_THREADS) void k_gemm(const uint16_t *__restrict__ A, const uint16_t *__restrict__ B,
                                                        float *C, uint16_t *Cb, int M, int N, int K, long sab,
                                                        long sam, long sak, long sbb, long sbk, long sbn, long scb,
                                                        long scm, long scn, float alpha, float beta) {
-  __shared__ uint16_t As[TM][TK + 2];
-  __shared__ uint16_t Bs[TK][TN + 2];
+  // A tile by row (m, k) and B tile by column (n, k): a lane's eight K values
+  // of either operand are one 16-byte LDS read
+  __shared__ __align__(16) uint16_t As[TM][TKP];
+  __shared__ __align__(16) uint16_t Bs[TN][TKP];
   const int b = blockIdx.z;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   A += (long)b * sab;
   B += (long)b * sbb;
+  // staging: each thread one row (column) and one group of eight K values.
+  // When the matrix is contiguous along M (N), consecutive threads take
+  // consecutive rows, so each of the eight element loads is coalesced;
+  // otherwise four threads cover one row's 32 K values.
+  const bool a_rows = sam == 1 && sak != 1;
+  const int am = a_rows ? tid & 63 : tid >> 2, ak = a_rows ? (tid >> 6) * 8 : (tid & 3) * 8;
+  const bool b_cols = sbn == 1 && sbk != 1;
+  const int bn = b_cols ? tid & 63 : tid >> 2, bk = b_cols ? (tid >> 6) * 8 : (tid & 3) * 8;
+  const int gm = m0 + am, hn = n0 + bn;
+  const long abase = (long)gm * sam, bbase = (long)hn * sbn;
   f32x16 acc;
 #pragma unroll
   for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
   for (int k0 = 0; k0 < K; k0 += TK) {
-    // stage A (TM x TK) and B (TK x TN): 1024 elements each, 4 per thread
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int idx = tid + e * GEMM_THREADS;
-      const int am = idx / TK, ak = idx % TK;
-      const int gm = m0 + am, gk = k0 + ak;
-      As[am][ak] = (gm < M && gk < K) ? A[(long)gm * sam + (long)gk * sak] : (uint16_t)0;
-      const int bk = idx / TN, bn = idx % TN;
-      const int hk = k0 + bk, hn = n0 + bn;
-      Bs[bk][bn] = (hk < K && hn < N) ? B[(long)hk * sbk + (long)hn * sbn] : (uint16_t)0;
-    }
+    const uint4 av = load8(A, abase, sak, gm < M, k0 + ak, K);
+    const uint4 bv = load8(B, bbase, sbk, hn < N, k0 + bk, K);
+    *(uint4 *)&As[am][ak] = av;
+    *(uint4 *)&Bs[bn][bk] = bv;
     __syncthreads();
+    // lane l holds A(row l % 32, k 8 (l / 32) + j) and B(k 8 (l / 32) + j, col l % 32), j = 0..7
+    const int r = lane & 31, kq = 8 * (lane >> 5);
 #pragma unroll
-    for (int ks = 0; ks < TK; ks += 8) {
-      // lane l holds A(row l % 32, k 4 (l / 32) + j) and B(k 4 (l / 32) + j, col l % 32)
-      bf16x4 af, bfr;
-      const int r = lane & 31, kq = ks + 4 * (lane >> 5);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        af[j] = (short)As[wm + r][kq + j];
-        bfr[j] = (short)Bs[kq + j][wn + r];
-      }
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(af, bfr, acc, 0, 0, 0);
+    for (int ks = 0; ks < TK; ks += 16) {
+      const bf16x8 af = *(const bf16x8 *)&As[wm + r][ks + kq];
+      const bf16x8 bfr = *(const bf16x8 *)&Bs[wn + r][ks + kq];
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af, bfr, acc, 0, 0, 0);
     }
     __syncthreads();
   }

@@ -109,10 +109,12 @@ struct ccp_sim {
   float contact_rc = 0;
   uint32_t cbins = 0, cbin_size = 0, cinterval = 0;
   uint32_t graph_steps = 64;
+  bool use_graphs = true; // false: every step launched from the host (profilers that cannot follow graphs)
   uint32_t reach = 1; // neighbour cells scanned on each side
   uint32_t lanes = 3; // lanes per bead of the force kernel
   bool integrate_pass = false; // 8 lanes: bonds, wall and stage update in a one-lane-per-bead pass
   bool group_types = true;     // 8 lanes: het beads first within each workgroup
+  bool typed = false;          // 8 lanes at reach 2: type-ordered work list (k_force_typed)
   uint32_t rounds = 1;         // 8 lanes: pair-phase rounds per wavefront before the per-bead work
   float fcap = 0.0f;
   KParams kp{};
@@ -141,27 +143,54 @@ struct ccp_sim {
     GridGeom g{};
     DevBuf<uint32_t> count, bs0, bs1, cstart, soi;
     DevBuf<float4> bucket, spos; // bucket: members' positions, index in the w bits
+    // type-ordered work list and the per-block het totals of the binning
+    // (double buffered as bs0/bs1), for the step grid of k_force_typed
+    DevBuf<uint32_t> hbs0, hbs1, work;
     int clean = 0;       // which per-block total buffer is zero
     bool valid = false;  // holds the cell list of the current positions
     bool allocated = false;
     uint32_t *bs(int k) { return k ? bs1.p : bs0.p; }
-    void alloc(size_t n_obj) {
+    uint32_t *hbs(int k) { return k ? hbs1.p : hbs0.p; }
+    void alloc(size_t n_obj, bool typed = false) {
       count.alloc(g.n_c); bs0.alloc(g.nblk); bs1.alloc(g.nblk);
       bucket.alloc((size_t)g.n_c * g.cap); cstart.alloc(g.n_c + 1); soi.alloc(n_obj); spos.alloc(n_obj);
       hip_check(hipMemset(count.p, 0, (size_t)g.n_c * 4));
       hip_check(hipMemset(bs0.p, 0, (size_t)g.nblk * 4));
       hip_check(hipMemset(bs1.p, 0, (size_t)g.nblk * 4));
       hip_check(hipMemset(cstart.p, 0, ((size_t)g.n_c + 1) * 4));
+      if (typed) {
+        hbs0.alloc(g.nblk); hbs1.alloc(g.nblk); work.alloc(n_obj);
+        hip_check(hipMemset(hbs0.p, 0, (size_t)g.nblk * 4));
+        hip_check(hipMemset(hbs1.p, 0, (size_t)g.nblk * 4));
+        // every entry a valid slot from the start
+        hip_check(hipMemset(work.p, 0, std::max<size_t>(n_obj, 1) * 4));
+      }
       clean = 0;
       valid = false;
       allocated = true;
     }
     void free_() {
-      DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &cstart, &soi};
+      DevBuf<uint32_t> *u[] = {&count, &bs0, &bs1, &cstart, &soi, &hbs0, &hbs1, &work};
       for (auto *b : u) b->free_();
       bucket.free_();
       spos.free_();
       allocated = false;
+    }
+    // the ordering job of this grid's buffers; with the work list when the
+    // grid has one and `typed` asks for it (n_het het beads of n_slots)
+    OrderJob order_job(const GridGeom &geom, bool typed, uint32_t n_het, uint32_t n_slots, float4 *out_spos,
+                       uint32_t *out_cstart, uint32_t *out_soi) {
+      OrderJob j{geom, count.p, bs(clean), bs(clean ^ 1), bucket.p, out_spos, out_soi, out_cstart};
+      if (work.p) {
+        j.hbs_clear = hbs(clean ^ 1);
+        if (typed) {
+          j.work = work.p;
+          j.hbs_read = hbs(clean);
+          j.n_het = n_het;
+          j.n_slots = n_slots;
+        }
+      }
+      return j;
     }
   };
   HostGrid gf; // fine grid of every bead
@@ -237,7 +266,7 @@ struct ccp_sim {
     return k;
   }
   void build_grid(HostGrid &G, const GridGeom &geom, const float4 *pos, int n_obj, bool het_only, float4 *out_spos,
-                  uint32_t *out_cstart, uint32_t *out_soi);
+                  uint32_t *out_cstart, uint32_t *out_soi, bool work_list = false);
   bool take_overflow();
   void grow_capacity();
   void ensure_grids();

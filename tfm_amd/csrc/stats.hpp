@@ -12,9 +12,9 @@ namespace ccpmi {
 
 struct SeriesStat {
   double avg = 0.0, var = 0.0, sem = 0.0;
-  uint32_t f_n_b = 0; // final number of blocks of the blocking method
-  uint32_t i_t = 0;   // thermalisation index (marginal standard error rule)
-  bool thm = false;   // thermalised: i_t differs from the half-length cut
+  uint32_t blocks = 0;  // number of blocks where the blocking method stopped
+  uint32_t cut = 0;     // first value kept (marginal standard error rule)
+  bool settled = false; // the cut differs from the half-length one
 };
 
 // Mean, unbiased variance and independent-sample error (calc_stats on a
@@ -38,7 +38,7 @@ inline SeriesStat independent_stats(const float *v, size_t n) {
 // Error of the mean of a correlated series by the blocking method
 // (Flyvbjerg-Petersen), stopping once the variance of the mean no longer
 // rises beyond its own uncertainty (sostat.cu:96-138). ccp leaves sem and
-// f_n_b unset for fewer than 4 values; here they keep the independent
+// the block count unset for fewer than 4 values; here they keep the independent
 // estimate and n.
 inline SeriesStat blocking_stats(const float *v, size_t n) {
   std::vector<float> av(v, v + n);
@@ -47,50 +47,50 @@ inline SeriesStat blocking_stats(const float *v, size_t n) {
   s.avg = ind.avg;
   s.var = ind.var;
   s.sem = ind.sem;
-  s.f_n_b = (uint32_t)n;
-  size_t n_e = av.size();
-  double ivm = ind.var / n_e;
-  double ulivm = ivm * (1.0 + std::sqrt(2.0 / (n_e - 1.0)));
-  while (n_e >= 4) {
-    size_t i_b = 0;
-    for (; 2 * i_b + 1 < n_e; ++i_b) av[i_b] = 0.5 * (av[2 * i_b] + av[2 * i_b + 1]);
-    n_e = i_b;
-    av.resize(n_e);
-    ind = independent_stats(av.data(), n_e);
-    ivm = ind.var / n_e;
-    s.sem = std::sqrt(ivm);
-    s.f_n_b = (uint32_t)n_e;
-    if (ivm > ulivm)
-      ulivm = ivm * (1.0 + std::sqrt(2.0 / (n_e - 1.0)));
+  s.blocks = (uint32_t)n;
+  size_t len = av.size();
+  double var_mean = ind.var / len;
+  double bound = var_mean * (1.0 + std::sqrt(2.0 / (len - 1.0)));
+  while (len >= 4) {
+    size_t half = 0;
+    for (; 2 * half + 1 < len; ++half) av[half] = 0.5 * (av[2 * half] + av[2 * half + 1]);
+    len = half;
+    av.resize(len);
+    ind = independent_stats(av.data(), len);
+    var_mean = ind.var / len;
+    s.sem = std::sqrt(var_mean);
+    s.blocks = (uint32_t)len;
+    if (var_mean > bound)
+      bound = var_mean * (1.0 + std::sqrt(2.0 / (len - 1.0)));
     else
       break;
   }
   return s;
 }
 
-// Time-series statistics (sostat.cu:141-183): the thermalisation cut i_t is
+// Time-series statistics (sostat.cu:141-183): the thermalisation cut is
 // the one of n/2, n/4, ..., n/64 with the smallest marginal standard error
 // var (m - 1) / m^2 of the remaining m values; the series counts as
 // thermalised unless the cut stays at n/2. Statistics of the values after
 // the cut by the blocking method. With no finite marginal error (fewer than
 // two values after every cut) the cut stays at n/2, where ccp leaves it unset.
 inline SeriesStat series_stats(const float *v, size_t n) {
-  double min_mse = INFINITY;
-  uint32_t i_t = (uint32_t)(n / 2);
+  double best = INFINITY;
+  uint32_t keep_from = (uint32_t)(n / 2);
   for (uint32_t d = 2; d < 128; d *= 2) {
-    const uint32_t cut = (uint32_t)(n / d);
-    const size_t m = n - cut;
-    const SeriesStat ind = independent_stats(v + cut, m);
+    const uint32_t from = (uint32_t)(n / d);
+    const size_t m = n - from;
+    const SeriesStat ind = independent_stats(v + from, m);
     // m * m in 32-bit unsigned arithmetic, as ccp (wraps above 65535 values)
     const double mse = ind.var * (double)(uint32_t)(m - 1) / (double)((uint32_t)m * (uint32_t)m);
-    if (mse < min_mse) {
-      min_mse = mse;
-      i_t = cut;
+    if (mse < best) {
+      best = mse;
+      keep_from = from;
     }
   }
-  SeriesStat s = blocking_stats(v + i_t, n - i_t);
-  s.i_t = i_t;
-  s.thm = i_t != n / 2;
+  SeriesStat s = blocking_stats(v + keep_from, n - keep_from);
+  s.cut = keep_from;
+  s.settled = keep_from != n / 2;
   return s;
 }
 

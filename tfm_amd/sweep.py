@@ -289,6 +289,7 @@ def run(argv=None):
         "plain_one_by_one": {"sets_per_hr": plain, "fraction_within_bar": bd_ok, "resolved_per_hr": plain_resolved},
         "vs_plain": (guided / plain_resolved) if plain_resolved > 0 else None,
         "scored_per_hr": a.candidates / (t_total / 3600.0),
+        "top_errors": [float(e) for e in np.sort(top_err)],
         "contact_map_error": {"surrogate_top": float(top_err.mean()), "surrogate_validation": val_err,
                               "mean_map_baseline": base_err, "bd_sampling_noise": noise_err,
                               "near": {"max_separation_bins": NEAR, "surrogate_top": float(top_near.mean()),
