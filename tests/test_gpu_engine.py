@@ -277,7 +277,7 @@ def test_type_grouping_is_bit_exact():
     b.flags = fl
     b.positions = xyz
     # the type-ordered work list: the het beads keep the same lists and sums,
-    # the others run on four lanes over reach 1, another FP32 summation order
+    # the others run on two lanes over reach 1, another FP32 summation order
     c = tfm_amd.Simulation({**bench.PARAMS, **shape, "type_order": 1})
     c.flags = fl
     c.positions = xyz
@@ -286,9 +286,26 @@ def test_type_grouping_is_bit_exact():
         s.run_steps(300)
         s.synchronize()
     assert np.array_equal(a.positions, b.positions)
-    dev = np.abs(c.positions.astype(np.float64) - a.positions).max()
+    # one force evaluation of the same state by both kernels: the het beads'
+    # sums are the same bits; the others differ by the FP32 rounding of
+    # another order of the same terms (a few tens of terms, each at most
+    # max|f|, rounded at 2^-24: a bound of 1e-5 max|f| holds with a margin)
+    state = c.positions
+    a.positions = state
+    fa = a.compute_forces(po.FF_LJ)
+    fc = c.compute_forces(po.FF_LJ)
+    het = (fl & po.HET) != 0
+    assert np.array_equal(fa[het], fc[het])
+    scale = float(np.abs(fa).max())
+    ferr = float(np.abs(fa[~het].astype(np.float64) - fc[~het]).max())
+    step(f"  forces of one state: het beads bit-exact, others max deviation {ferr:.3g} of max|f| {scale:.3g}")
+    assert ferr <= 1e-5 * scale
+    # the trajectories themselves drift apart as the rounding differences
+    # grow in the chaotic dynamics (~1e-3 after 300 steps against steps of
+    # ~0.05 sigma); a wrong pair set would move beads by ~0.1 sigma or more
+    dev = np.abs(state.astype(np.float64) - b.positions).max()
     step(f"  work list vs one kernel after 300 steps: max deviation {dev:.3g}")
-    assert dev < 1e-3
+    assert dev < 1e-2
     for s in (a, b, c):
         s.close()
 

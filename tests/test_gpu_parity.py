@@ -72,6 +72,7 @@ FORCE_CASES = [
     # the type-ordered work list with ccp's force field (1.25 sigma cells keep
     # its short cutoff within one cell)
     dict(ff=po.FF_CCP, n_l=40, cell_reach=2, lanes_per_bead=8, cell_size=1.25),
+    dict(ff=po.FF_ICG, n_l=0, cell_reach=2, lanes_per_bead=8, cell_size=1.25),
     # eight lanes per bead, the pair phase of 2, 4 or 8 groups of eight beads
     # per wavefront before the per-bead work
     dict(ff=po.FF_LJ, n_l=30, eps_het=0.6, eps_lam=1.2, cell_reach=2, lanes_per_bead=8, bal_rounds=2),
@@ -234,29 +235,33 @@ def test_step_grid_bit_exact():
     sim.close()
 
 
-@pytest.mark.parametrize("reach", [2, 1])
-def test_forces_crowded_lists(reach):
+@pytest.mark.parametrize("reach,het", [(2, True), (1, True), (2, False)])
+def test_forces_crowded_lists(reach, het):
     # Het beads on a cubic lattice of spacing 1 sigma (volume fraction 0.52):
     # the inner beads' trimmed neighbour rows hold more slots than the
     # eight-lane kernel's 128-entry list, so each of their lanes scans its own
     # rows directly (the crowded path), while the surface beads use the list;
-    # at reach 2 (1.25 sigma cells) and reach 1 (2.5 sigma).
-    step(f"forces, crowded lists, reach {reach}")
+    # at reach 2 (1.25 sigma cells) and reach 1 (2.5 sigma). Other beads
+    # (2^(1/6) sigma cutoff, two lanes each over reach 1 of the 1.25 sigma
+    # cells, lists of 48) on a lattice of spacing 0.75 sigma overflow theirs.
+    step(f"forces, crowded lists, reach {reach}, het {het}")
     k = 12
     N = k ** 3
+    spacing, cap, cut = (1.0, 128, 2.5) if het else (0.75, 48, 2.0 ** (1.0 / 6.0))
     rng = np.random.default_rng(5)
     idx = np.arange(N)
     x, y, z = idx % k, (idx // k) % k, idx // (k * k)
     # a snake through the lattice, so consecutive beads are one spacing apart
     x = np.where(y % 2 == 1, k - 1 - x, x)
     y = np.where(z % 2 == 1, k - 1 - y, y)
-    r = np.stack([x, y, z], axis=1).astype(np.float64) - (k - 1) / 2 + rng.uniform(-0.05, 0.05, (N, 3))
+    r = (np.stack([x, y, z], axis=1).astype(np.float64) - (k - 1) / 2) * spacing
+    r += rng.uniform(-0.05, 0.05, (N, 3))
     R_n = 14.0
-    fl = np.full(N, po.HET, np.uint8)
+    fl = np.full(N, po.HET if het else 0, np.uint8)
     sim, osys, r64 = make_pair(N, R_n, fl, r, ff=po.FF_LJ, eps_het=0.6, eps_lam=1.0, cell_reach=reach,
                                lanes_per_bead=8)
     # list length of every bead on the host: the slots of the cells whose
-    # nearest point lies within its cutoff (2.5 sigma + 1e-3)
+    # nearest point lies within its cutoff (+ 1e-3)
     csl = np.float32(sim.info.fine_csl)
     cps = sim.info.fine_cps
     r32 = r64.astype(np.float32)
@@ -273,18 +278,18 @@ def test_forces_crowded_lists(reach):
                 e0 = (c - cps // 2) * csl
                 d = max(e0 - r32[i, a], r32[i, a] - (e0 + csl), 0.0)
                 gap += d * d
-            if gap <= (2.5 + 1e-3) ** 2:
+            if gap <= (cut + 1e-3) ** 2:
                 tot += m
         lengths.append(tot)
     lengths = np.array(lengths)
     # the lattice's inner beads overflow the list, its surface beads do not
-    crowded = float((lengths > 128).mean())
-    assert crowded > 0.25 and lengths.min() <= 128, (crowded, lengths.min())
+    crowded = float((lengths > cap).mean())
+    assert crowded > 0.25 and lengths.min() <= cap, (crowded, lengths.min())
     got = sim.compute_forces(po.FF_LJ).astype(np.float64)
     want = osys.forces(r64, method=0)
     w32 = osys.forces(r64, method=0, prec=32)
     err, err32, scale = np.abs(got - want).max(), np.abs(w32 - want).max(), np.abs(want).max()
-    step(f"  lists over 128 entries: {crowded:.0%} of the sampled beads; max|f|={scale:.3g} gpu err={err:.3g} "
+    step(f"  lists over {cap} entries: {crowded:.0%} of the sampled beads; max|f|={scale:.3g} gpu err={err:.3g} "
          f"oracle fp32 err={err32:.3g}")
     assert err <= 3 * err32 + 1e-6 * scale
     sim.close()

@@ -150,7 +150,7 @@ void ccp_sim::configure(const ParMap &par) {
   if (rounds != 1 && rounds != 2 && rounds != 4 && rounds != 8) throw Error(CCP_ERR_PARAM, "bal_rounds out of range");
   // with 8 lanes at reach 2: the force kernel takes the beads from a
   // type-ordered work list (het beads first, eight lanes each; the others
-  // four lanes each over reach 1), when the short cutoffs of the pair set and
+  // two lanes each over reach 1), when the short cutoffs of the pair set and
   // of the relaxation stay within one cell side
   {
     const double margin = 1e-3, rs = lj ? std::pow(2.0, 1.0 / 6.0) : RCO;
@@ -1225,6 +1225,9 @@ int ccp_compute_forces(ccp_sim *h, int ff, float *f_xyz, size_t n) {
     q.flags = h->flags.p;
     q.step = h->dstep.p;
     q.fpair = h->fpair.p;
+    // the grid build above wrote the type-ordered work list: the forces come
+    // from the kernel the steps run
+    q.work = h->typed ? h->gf.work.p : nullptr;
     launch_force(h->stream, ff, 0, k, q);
     hip_check(hipGetLastError());
     std::vector<float4> out(h->N);

@@ -904,7 +904,7 @@ constexpr uint64_t ring2_word(int m) {
 constexpr uint64_t RING2_WORDS[3] = {ring2_word(0), ring2_word(1), ring2_word(2)};
 
 // Pair (and LBS) forces of the bead `me` over its trimmed candidate list,
-// spread over the bead's G lanes (sub = 0..G-1; G = 8, or 4 for beads whose
+// spread over the bead's G lanes (sub = 0..G-1; G = 8, or 2 for beads whose
 // cutoff stays within one cell at reach 1) and added to f; on return every
 // lane of the group holds the bead's total. `list` is the group's LDS list of
 // LCAP entries, `self` the bead's own cell-order slot; the caller's lanes of
@@ -913,9 +913,12 @@ template <int MODE, bool BATCH, int R, int G = C_LANES, int LCAP = C_LIST>
 __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, uint32_t *list, bool active,
                                              uint32_t self, int sub, int lane, float4 me, float &fx, float &fy,
                                              float &fz) {
-  static_assert(G == 8 || G == 4, "eight or four lanes per bead");
-  static_assert(G >= 2 * R + 1, "the group's lanes hold the gaps of offsets -R..R");
-  static_assert(R == 1 || G == 8, "reach 2 takes the ring order of eight lanes");
+  static_assert(G == 8 || (G == 2 && R == 1), "eight lanes per bead, or two at reach 1");
+  // eight lanes: the group's lanes hold the y and z gaps of offsets -R..R
+  // and each row fetches its two; two lanes compute all three themselves
+  constexpr bool LANE_GAPS = G >= 2 * R + 1;
+  // reach 2: rows in ring order, the outer ring skippable
+  constexpr bool RING = R == 2;
   // NROW: the rows other than the centre one, RPL of them per lane
   constexpr int W = 2 * R + 1, NROW = W * W - 1, RPL = (NROW + G - 1) / G;
   const uint32_t w = wbits(me);
@@ -930,8 +933,26 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   // y and z: lane sub of the bead's group holds the squared gaps to the
   // cells d = sub - R steps away; each row takes its two from those lanes
   const int d = sub - R, gb4 = (lane & ~(G - 1)) << 2;
-  const float gyl = axis_gap2(ry - (float)(cy - p.gf.half) * csl, csl, cy, d, cps),
-              gzl = axis_gap2(rz - (float)(cz - p.gf.half) * csl, csl, cz, d, cps);
+  float gyl = 0.0f, gzl = 0.0f;
+  AxisGaps<R> GY, GZ;
+  if constexpr (LANE_GAPS) {
+    gyl = axis_gap2(ry - (float)(cy - p.gf.half) * csl, csl, cy, d, cps);
+    gzl = axis_gap2(rz - (float)(cz - p.gf.half) * csl, csl, cz, d, cps);
+  } else {
+    GY = axis_gaps<R>(ry - (float)(cy - p.gf.half) * csl, csl, cy, cps);
+    GZ = axis_gaps<R>(rz - (float)(cz - p.gf.half) * csl, csl, cz, cps);
+  }
+  // squared gap to the cells i - R steps away (i = 0 .. 2R) from the
+  // bead's own gaps
+  auto own_gap = [](const AxisGaps<R> &A, int i) {
+    float g = 0.0f;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      g = i == R - 1 - k ? A.lo[k] : g;
+      g = i == R + 1 + k ? A.hi[k] : g;
+    }
+    return g;
+  };
   // cell (cx, cy + iy - R, cz + iz - R) is base0 + (iz cps + iy) cps; the
   // products take the 24-bit multiplier: grids are set up with cps^3 < 2^31,
   // so cps and iz cps + iy stay below 2^24
@@ -941,7 +962,7 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   // wavefront in which no bead's cutoff reaches a cell two steps away in y
   // or z.
   bool any_outer = true;
-  if constexpr (R == 2) {
+  if constexpr (RING) {
     const float m2 = fminf(gyl, gzl);
     const float near2 = fminf(group_lane(m2, gb4, 0), group_lane(m2, gb4, 2 * R));
     any_outer = __ballot(active && rc2 >= near2) != 0ull;
@@ -970,7 +991,7 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   for (int m = 0; m < RPL; ++m) {
     o0[m] = 0u;
     o1[m] = 0u;
-    if (R == 2 && m >= 1 && !any_outer) continue;
+    if (RING && m >= 1 && !any_outer) continue;
     // Row (dy, dz) = (iy - R, iz - R): its cells [c0, c1) that can hold a
     // bead within sqrt(rc2), empty when the row's y-z slab is farther (or
     // outside the grid), else the run around cx whose x gaps fit in the rest,
@@ -979,12 +1000,17 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     // row r of the W x W rows with the centre (index W * W / 2) left out
     const int rr = r < W * W / 2 ? r : r + 1;
     int iy = rr % W, iz = rr / W;
-    if constexpr (R == 2) {
+    if constexpr (RING) {
       const uint32_t v = (uint32_t)(RING2_WORDS[m] >> (6 * sub)) & 63u;
       iy = (int)(v & 7u);
       iz = (int)(v >> 3);
     }
-    const float h2 = rc2 - group_lane(gyl, gb4, iy) - group_lane(gzl, gb4, iz);
+    float h2;
+    if constexpr (LANE_GAPS) {
+      h2 = rc2 - group_lane(gyl, gb4, iy) - group_lane(gzl, gb4, iz);
+    } else {
+      h2 = rc2 - own_gap(GY, iy) - own_gap(GZ, iz);
+    }
     const bool ok = active && r < NROW && h2 >= 0.0f;
     uint32_t t = __umul24((uint32_t)iz, (uint32_t)cps) + (uint32_t)iy;
     asm("" : "+v"(t));
@@ -1037,13 +1063,15 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   {
     const uint32_t v1 = dpp_u<DPP_SHR1>(incl);
     incl += sub >= 1 ? v1 : 0u;
-    const uint32_t v2 = dpp_u<DPP_SHR2>(incl);
-    incl += sub >= 2 ? v2 : 0u;
+    if constexpr (G >= 4) {
+      const uint32_t v2 = dpp_u<DPP_SHR2>(incl);
+      incl += sub >= 2 ? v2 : 0u;
+    }
     if constexpr (G == 8) incl += dpp_u<DPP_SHR4, 0xA>(incl);
   }
   uint32_t total = tot;
   total += dpp_u<DPP_XOR1>(total);
-  total += dpp_u<DPP_XOR2>(total);
+  if constexpr (G >= 4) total += dpp_u<DPP_XOR2>(total);
   if constexpr (G == 8) total += dpp_u<DPP_MIRROR8>(total);
   // the centre row's entries come first
   total += clen;
@@ -1117,7 +1145,9 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   // the half-row mirror (lane 7 - i), which pairs each quad with the other;
   // every lane adds the same two values each time, so all end equal
   fx += dpp_f<DPP_XOR1>(fx); fy += dpp_f<DPP_XOR1>(fy); fz += dpp_f<DPP_XOR1>(fz);
-  fx += dpp_f<DPP_XOR2>(fx); fy += dpp_f<DPP_XOR2>(fy); fz += dpp_f<DPP_XOR2>(fz);
+  if constexpr (G >= 4) {
+    fx += dpp_f<DPP_XOR2>(fx); fy += dpp_f<DPP_XOR2>(fy); fz += dpp_f<DPP_XOR2>(fz);
+  }
   if constexpr (G == 8) {
     fx += dpp_f<DPP_MIRROR8>(fx); fy += dpp_f<DPP_MIRROR8>(fy); fz += dpp_f<DPP_MIRROR8>(fz);
   }
@@ -1246,31 +1276,40 @@ __global__ __launch_bounds__(BAL_BLOCK) void k_force_bal(KParams p, KPtrs q) {
 // written by the order pass). Every wavefront holds beads of one type:
 // - the first p.het_blocks workgroups take the het beads, eight lanes per
 //   bead over the 5 x 5 x 5 half-size cells (their cutoff spans two cells);
-// - the rest take the others, four lanes per bead over 3 x 3 x 3 of the same
+// - the rest take the others, two lanes per bead over 3 x 3 x 3 of the same
 //   cells (their cutoff is shorter than one cell side, checked by the host),
-//   32 beads per workgroup.
+//   64 beads per workgroup with lists of up to T_LIST entries. Their lists
+//   hold ~19 candidates, so two lanes keep the gathers in rounds while each
+//   bead's set-up (rows, list, epilogue) is repeated on two lanes, not four
+//   or eight.
 // Each part's workgroups are renumbered per XCD on their own, so every XCD
 // gets its share of both. Results are those of k_force_bal: the same pair
 // set, summed in another order.
-constexpr int T_LIST_SHORT = 64; // list entries per bead of the four-lane part
+constexpr int H_LANES = C_LANES;                       // lanes per bead of the het part
+constexpr int H_BEADS_PER_BLOCK = BAL_BLOCK / H_LANES; // 16
+constexpr int T_LANES = 2;                             // lanes per bead of the non-het part
+constexpr int T_BEADS_PER_BLOCK = BAL_BLOCK / T_LANES; // 64
+constexpr int T_LIST = 48;                             // list entries per bead of the non-het part
+constexpr int T_LDS = H_BEADS_PER_BLOCK * C_LIST > T_BEADS_PER_BLOCK * T_LIST ? H_BEADS_PER_BLOCK * C_LIST
+                                                                              : T_BEADS_PER_BLOCK * T_LIST;
+static_assert(H_BEADS_PER_BLOCK == TYPED_HET_BEADS_PER_BLOCK, "het workgroups as the host sizes them");
 template <int MODE, int STAGE, bool BATCH>
 __global__ __launch_bounds__(BAL_BLOCK) void k_force_typed(KParams p, KPtrs q) {
-  __shared__ uint32_t s_list[C_BEADS_PER_BLOCK * C_LIST];
-  static_assert(C_BEADS_PER_BLOCK * C_LIST == 2 * C_BEADS_PER_BLOCK * T_LIST_SHORT, "both parts share the list");
+  __shared__ uint32_t s_list[T_LDS];
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const int b = (int)blockIdx.x;
   const bool het_part = b < p.het_blocks;
   int t, sub, grp;
   uint32_t end;
   if (het_part) {
-    sub = lane % C_LANES;
-    grp = wave * (64 / C_LANES) + lane / C_LANES;
-    t = xcd_remap(b, p.het_blocks) * C_BEADS_PER_BLOCK + grp;
+    sub = lane % H_LANES;
+    grp = wave * (64 / H_LANES) + lane / H_LANES;
+    t = xcd_remap(b, p.het_blocks) * H_BEADS_PER_BLOCK + grp;
     end = (uint32_t)p.n_het;
   } else {
-    sub = lane % 4;
-    grp = wave * 16 + lane / 4;
-    t = p.n_het + xcd_remap(b - p.het_blocks, (int)gridDim.x - p.het_blocks) * 2 * C_BEADS_PER_BLOCK + grp;
+    sub = lane % T_LANES;
+    grp = wave * (64 / T_LANES) + lane / T_LANES;
+    t = p.n_het + xcd_remap(b - p.het_blocks, (int)gridDim.x - p.het_blocks) * T_BEADS_PER_BLOCK + grp;
     end = (uint32_t)p.N;
   }
   const bool in = (uint32_t)t < end;
@@ -1296,11 +1335,11 @@ __global__ __launch_bounds__(BAL_BLOCK) void k_force_typed(KParams p, KPtrs q) {
     }
   }
   if (het_part)
-    bal_pair_sum<MODE, BATCH, 2, C_LANES, C_LIST>(p, q, s_list + grp * C_LIST, active, slot, sub, lane, me, fx, fy,
+    bal_pair_sum<MODE, BATCH, 2, H_LANES, C_LIST>(p, q, s_list + grp * C_LIST, active, slot, sub, lane, me, fx, fy,
                                                    fz);
   else
-    bal_pair_sum<MODE, BATCH, 1, 4, T_LIST_SHORT>(p, q, s_list + grp * T_LIST_SHORT, active, slot, sub, lane, me,
-                                                   fx, fy, fz);
+    bal_pair_sum<MODE, BATCH, 1, T_LANES, T_LIST>(p, q, s_list + grp * T_LIST, active, slot, sub, lane, me, fx, fy,
+                                                  fz);
   if (!active || !lead) return;
   stage_update<STAGE>(p, q, i, w, me, fx, fy, fz, inp);
 }
@@ -1867,8 +1906,8 @@ template <int MODE, bool BATCH, int R>
 static void force_balanced(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
   if constexpr (R == 2) {
     if (p.typed && q.work) {
-      // het beads in het_blocks workgroups of 16, the others in workgroups of 32
-      const unsigned gt = (unsigned)p.het_blocks + blocks_for(p.N - p.n_het, 2 * C_BEADS_PER_BLOCK);
+      // het beads in het_blocks workgroups of 16, the others in workgroups of 64
+      const unsigned gt = (unsigned)p.het_blocks + blocks_for(p.N - p.n_het, T_BEADS_PER_BLOCK);
       if (stage == 0) k_force_typed<MODE, 0, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q);
       else if (stage == 1) k_force_typed<MODE, 1, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q);
       else k_force_typed<MODE, 2, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q);

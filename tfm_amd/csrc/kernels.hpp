@@ -74,7 +74,7 @@ struct KParams {
   int rounds;  // 8 lanes: beads per lane group of a wavefront (1, 2, 4 or 8), eight per round of the pair phase
   // 8 lanes at reach 2 over a type-ordered work list (k_force_typed): the
   // het beads (n_het of them) in the first het_blocks workgroups, eight lanes
-  // each; the others four lanes each over reach 1 of the same cells
+  // each; the others two lanes each over reach 1 of the same cells
   int typed, n_het, het_blocks;
   int frozen;  // batches: the bead's system is frozen (set per bead from its SysParams)
   // nucleus geometry (cngeom, chrdat.cuh:48-69)
@@ -176,12 +176,16 @@ struct BondJob {
   int n_sys;
 };
 
+// Het beads per 128-lane workgroup of k_force_typed (eight lanes each).
+constexpr int TYPED_HET_BEADS_PER_BLOCK = 16;
+
 // Het count of the type-ordered work list and the workgroups of its het part
-// (16 beads each, rounded to a multiple of the 8 XCDs so that the other part
-// starts on XCD 0).
+// (TYPED_HET_BEADS_PER_BLOCK beads each, rounded to a multiple of the 8 XCDs
+// so that the other part starts on XCD 0).
 inline void set_het_count(KParams &k, uint32_t n_het) {
+  constexpr uint32_t per = (uint32_t)TYPED_HET_BEADS_PER_BLOCK;
   k.n_het = (int)n_het;
-  k.het_blocks = (int)(((n_het + 15u) / 16u + 7u) / 8u * 8u);
+  k.het_blocks = (int)(((n_het + per - 1u) / per + 7u) / 8u * 8u);
 }
 
 // Launchers (kernels.hip). All are asynchronous on `s`.
