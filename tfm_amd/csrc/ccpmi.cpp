@@ -275,6 +275,8 @@ void ccp_sim::fill_kparams() {
 void ccp_sim::allocate() {
   hip_check(hipGetDevice(&dev));
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  hip_check(hipHostMalloc(reinterpret_cast<void **>(&hflags), FLAG_COUNT * sizeof(uint32_t), hipHostMallocDefault));
   r.alloc(N); er.alloc(N); pstar.alloc(N); aux.alloc(N); znoise.alloc(N);
   if (bond_pass()) fbond.alloc(N);
   if (lanes == 8 && integrate_pass) fpair.alloc(N);
@@ -347,10 +349,15 @@ void ccp_sim::download_positions() {
   }
 }
 
+// Waits for everything enqueued on the stream, then raises any error flag:
+// the flags are copied into pinned host memory behind the stream's work and
+// one event wait (a spin, not an interrupt) covers both.
 void ccp_sim::check_flags() {
+  hip_check(hipMemcpyAsync(hflags, flags.p, FLAG_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  hip_check(hipEventRecord(done, stream));
+  hip_check(hipEventSynchronize(done));
   uint32_t f[FLAG_COUNT];
-  hip_check(hipMemcpyAsync(f, flags.p, sizeof(f), hipMemcpyDeviceToHost, stream));
-  hip_check(hipStreamSynchronize(stream));
+  std::memcpy(f, hflags, sizeof(f));
   if (f[FLAG_OVERFLOW] || f[FLAG_OUTSIDE] || f[FLAG_NONFINITE]) {
     hip_check(hipMemset(flags.p, 0, sizeof(f)));
     if (f[FLAG_NONFINITE]) throw Error(CCP_ERR_STATE, "non-finite bead position");
@@ -1114,10 +1121,7 @@ int ccp_run_steps(ccp_sim *h, uint64_t n) {
 }
 
 int ccp_synchronize(ccp_sim *h) {
-  return guarded(h, [&] {
-    hip_check(hipStreamSynchronize(h->stream));
-    h->check_flags();
-  });
+  return guarded(h, [&] { h->check_flags(); });
 }
 
 int ccp_get_time(const ccp_sim *h, uint32_t *i_f, float *t, uint64_t *step) {

@@ -136,6 +136,10 @@ struct ccp_sim {
   hipStream_t stream = nullptr;
   DevBuf<float4> r, er, pstar, lr, lspos, aux, fpair, znoise, fbond;
   DevBuf<uint32_t> lstart, flags, cmap;
+  // pinned host copy of the error flags, and the event a synchronization
+  // waits on (created without hipEventBlockingSync: the wait spins)
+  uint32_t *hflags = nullptr;
+  hipEvent_t done = nullptr;
   DevBuf<uint64_t> dstep;
   DevBuf<unsigned long long> dcount;
   // one uniform grid: binning buffers + ordered cell list
@@ -226,6 +230,8 @@ struct ccp_sim {
     gx.free_();
     dstep.free_();
     dcount.free_();
+    if (hflags) (void)hipHostFree(hflags);
+    if (done) (void)hipEventDestroy(done);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
