@@ -1283,8 +1283,10 @@ __global__ __launch_bounds__(BAL_BLOCK) void k_force_bal(KParams p, KPtrs q) {
 //   bead's set-up (rows, list, epilogue) is repeated on two lanes, not four
 //   or eight.
 // Each part's workgroups are renumbered per XCD on their own, so every XCD
-// gets its share of both. Results are those of k_force_bal: the same pair
-// set, summed in another order.
+// gets its share of both. After the pair sums, the first wavefront of a
+// workgroup runs the stage update and binning of both wavefronts' beads.
+// Results are those of k_force_bal: the same pair set, summed in another
+// order.
 constexpr int H_LANES = C_LANES;                       // lanes per bead of the het part
 constexpr int H_BEADS_PER_BLOCK = BAL_BLOCK / H_LANES; // 16
 constexpr int T_LANES = 2;                             // lanes per bead of the non-het part
@@ -1293,9 +1295,12 @@ constexpr int T_LIST = 48;                             // list entries per bead 
 constexpr int T_LDS = H_BEADS_PER_BLOCK * C_LIST > T_BEADS_PER_BLOCK * T_LIST ? H_BEADS_PER_BLOCK * C_LIST
                                                                               : T_BEADS_PER_BLOCK * T_LIST;
 static_assert(H_BEADS_PER_BLOCK == TYPED_HET_BEADS_PER_BLOCK, "het workgroups as the host sizes them");
+static_assert(BAL_BLOCK == 128, "the epilogue hand-over pairs the workgroup's two wavefronts");
+static_assert(8 * C_LIST * 4 >= 8 * 3 * 16 && 32 * T_LIST * 4 >= 32 * 3 * 16,
+              "a wavefront's list memory holds the hand-over records of its beads");
 template <int MODE, int STAGE, bool BATCH>
 __global__ __launch_bounds__(BAL_BLOCK) void k_force_typed(KParams p, KPtrs q) {
-  __shared__ uint32_t s_list[T_LDS];
+  __shared__ __align__(16) uint32_t s_list[T_LDS];
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const int b = (int)blockIdx.x;
   const bool het_part = b < p.het_blocks;
@@ -1340,8 +1345,37 @@ __global__ __launch_bounds__(BAL_BLOCK) void k_force_typed(KParams p, KPtrs q) {
   else
     bal_pair_sum<MODE, BATCH, 1, T_LANES, T_LIST>(p, q, s_list + grp * T_LIST, active, slot, sub, lane, me, fx, fy,
                                                   fz);
-  if (!active || !lead) return;
-  stage_update<STAGE>(p, q, i, w, me, fx, fy, fz, inp);
+  // The epilogue of both wavefronts' beads runs on the first one, so the
+  // per-bead work is issued once per workgroup instead of once per
+  // wavefront: the second hands over each bead's total, slot, position and
+  // stage input through its own (finished) part of the list memory, and
+  // lane 1 of each group of the first takes the bead of the same group of
+  // the second.
+  const int gw = lane / (het_part ? H_LANES : T_LANES);
+  float4 *xch = reinterpret_cast<float4 *>(s_list + (het_part ? 8 * C_LIST : 32 * T_LIST));
+  if (wave == 1 && lead) {
+    xch[3 * gw] = make_float4(fx, fy, fz, __uint_as_float(active ? slot : 0xffffffffu));
+    xch[3 * gw + 1] = me;
+    xch[3 * gw + 2] = inp;
+  }
+  __syncthreads();
+  if (wave == 1) return;
+  bool run = active && lead;
+  float4 mm = me, in2 = inp;
+  if (sub == 1) {
+    const float4 F = xch[3 * gw];
+    const uint32_t s2 = __float_as_uint(F.w);
+    run = s2 != 0xffffffffu;
+    fx = F.x;
+    fy = F.y;
+    fz = F.z;
+    mm = xch[3 * gw + 1];
+    in2 = xch[3 * gw + 2];
+    if constexpr (BATCH) load_system(p, q, run ? s2 : 0u);
+  }
+  if (!run) return;
+  const uint32_t w2 = wbits(mm);
+  stage_update<STAGE>(p, q, w2 & ID_MASK, w2, mm, fx, fy, fz, in2);
 }
 
 // Bonds, wall and the Heun stage of every bead, one lane per bead in index
