@@ -494,17 +494,23 @@ __device__ __forceinline__ PairCtx pair_ctx(const KParams &p, float4 me) {
   return c;
 }
 
+// Kind of bead of a pair term, when the caller knows it: any, a bead with
+// no het-het pairs (non-het), or a het bead (every het partner is a
+// het-het pair).
+enum BeadKind : int { BEAD_ANY = 0, BEAD_SHORT = 1, BEAD_HET = 2 };
+
 // pair_full with the bead's operands from a PairCtx: the same operations in
-// the same order, so the same bits. SHORT: the bead has no het-het pairs
-// (a non-het bead), so the het-het tail is not tested at all.
-template <int MODE, bool SHORT = false>
+// the same order, so the same bits. For a known kind the het-het test is
+// one sign test (het bead) or none (non-het bead).
+template <int MODE, int KIND = BEAD_ANY>
 __device__ __forceinline__ void pair_term(const KParams &p, const PairCtx &c, float4 o, float &fx, float &fy,
                                           float &fz) {
+  static_assert(W_HET == 0x80000000u, "the het flag is the sign bit");
   const uint32_t wj = wbits(o);
   const float dx = c.rx - o.x, dy = c.ry - o.y, dz = c.rz - o.z;
   const float d2 = dx * dx + dy * dy + dz * dz;
   const bool nb = (wj & ID_MASK) - c.im1 > 2u;
-  const bool hhi = SHORT ? false : (wj & c.het) != 0u;
+  const bool hhi = KIND == BEAD_SHORT ? false : KIND == BEAD_HET ? (int32_t)wj < 0 : (wj & c.het) != 0u;
   float fac;
   if constexpr (MODE == MODE_CCP) {
     const float d = sqrtf(d2);
@@ -522,7 +528,7 @@ __device__ __forceinline__ void pair_term(const KParams &p, const PairCtx &c, fl
       const float u = fast_rcp(d2);
       const float s6 = u * u * u;
       const float base = s6 * fmaf(48.0f, s6, -24.0f) * u;
-      const float f = (SHORT ? c.e_wca : wca ? c.e_wca : c.e_het) * base;
+      const float f = (KIND == BEAD_SHORT ? c.e_wca : wca ? c.e_wca : c.e_het) * base;
       fx += f * dx;
       fy += f * dy;
       fz += f * dz;
@@ -917,9 +923,9 @@ constexpr uint64_t RING2_WORDS[3] = {ring2_word(0), ring2_word(1), ring2_word(2)
 // cutoff stays within one cell at reach 1) and added to f; on return every
 // lane of the group holds the bead's total. `list` is the group's LDS list of
 // LCAP entries, `self` the bead's own cell-order slot; the caller's lanes of
-// one wavefront run this together. SHORT: the beads have no het-het pairs
-// (the non-het part of the type-ordered kernel).
-template <int MODE, bool BATCH, int R, int G = C_LANES, int LCAP = C_LIST, bool SHORT = false>
+// one wavefront run this together. KIND: the beads' kind when the caller
+// knows it (the two parts of the type-ordered kernel).
+template <int MODE, bool BATCH, int R, int G = C_LANES, int LCAP = C_LIST, int KIND = BEAD_ANY>
 __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, uint32_t *list, bool active,
                                              uint32_t self, int sub, int lane, float4 me, float &fx, float &fy,
                                              float &fz) {
@@ -972,7 +978,8 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   // wavefront in which no bead's cutoff reaches a cell two steps away in y
   // or z.
   bool any_outer = true;
-  if constexpr (RING) {
+  // (het beads always reach it: their cutoff spans two cells)
+  if constexpr (RING && KIND != BEAD_HET) {
     const float m2 = fminf(gyl, gzl);
     const float near2 = fminf(group_lane(m2, gb4, 0), group_lane(m2, gb4, 2 * R));
     any_outer = __ballot(active && rc2 >= near2) != 0ull;
@@ -1119,16 +1126,16 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
 #pragma unroll
       for (int k = 0; k < 8; ++k) e[k] = at_byte(q.fspos, list[n + k * G]);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) pair_term<MODE, SHORT>(p, pc, e[k], fx, fy, fz);
+      for (int k = 0; k < 8; ++k) pair_term<MODE, KIND>(p, pc, e[k], fx, fy, fz);
     }
     for (; base + 4 * G <= total; base += 4 * G) {
       const uint32_t n = base + sub;
       const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, list[n + G]),
                    c = at_byte(q.fspos, list[n + 2 * G]), d = at_byte(q.fspos, list[n + 3 * G]);
-      pair_term<MODE, SHORT>(p, pc, a, fx, fy, fz);
-      pair_term<MODE, SHORT>(p, pc, b, fx, fy, fz);
-      pair_term<MODE, SHORT>(p, pc, c, fx, fy, fz);
-      pair_term<MODE, SHORT>(p, pc, d, fx, fy, fz);
+      pair_term<MODE, KIND>(p, pc, a, fx, fy, fz);
+      pair_term<MODE, KIND>(p, pc, b, fx, fy, fz);
+      pair_term<MODE, KIND>(p, pc, c, fx, fy, fz);
+      pair_term<MODE, KIND>(p, pc, d, fx, fy, fz);
     }
     const uint32_t rem = total - base, n = base + sub, s4 = self << 4;
     const bool wide = __ballot(rem > 3 * G) != 0ull;
@@ -1138,18 +1145,18 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
       const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, jb), c = at_byte(q.fspos, jc);
       float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (wide) d = at_byte(q.fspos, n + 3 * G < total ? list[n + 3 * G] : s4);
-      pair_term<MODE, SHORT>(p, pc, a, fx, fy, fz);
-      pair_term<MODE, SHORT>(p, pc, b, fx, fy, fz);
-      pair_term<MODE, SHORT>(p, pc, c, fx, fy, fz);
-      if (wide) pair_term<MODE, SHORT>(p, pc, d, fx, fy, fz);
+      pair_term<MODE, KIND>(p, pc, a, fx, fy, fz);
+      pair_term<MODE, KIND>(p, pc, b, fx, fy, fz);
+      pair_term<MODE, KIND>(p, pc, c, fx, fy, fz);
+      if (wide) pair_term<MODE, KIND>(p, pc, d, fx, fy, fz);
     }
   } else {
     // crowded neighbourhood: each lane scans its own rows, and every eighth
     // entry of the centre row
-    for (uint32_t k = sub; k < clen; k += G) pair_term<MODE, SHORT>(p, pc, q.fspos[clo + k], fx, fy, fz);
+    for (uint32_t k = sub; k < clen; k += G) pair_term<MODE, KIND>(p, pc, q.fspos[clo + k], fx, fy, fz);
 #pragma unroll
     for (int m = 0; m < RPL; ++m)
-      for (uint32_t k = 0; k < len[m]; ++k) pair_term<MODE, SHORT>(p, pc, q.fspos[lo[m] + k], fx, fy, fz);
+      for (uint32_t k = 0; k < len[m]; ++k) pair_term<MODE, KIND>(p, pc, q.fspos[lo[m] + k], fx, fy, fz);
   }
   // sum over the bead's eight lanes: quad swaps (lane ^ 1, lane ^ 2), then
   // the half-row mirror (lane 7 - i), which pairs each quad with the other;
@@ -1350,11 +1357,11 @@ __global__ __launch_bounds__(BAL_BLOCK) void k_force_typed(KParams p, KPtrs q) {
     }
   }
   if (het_part)
-    bal_pair_sum<MODE, BATCH, 2, H_LANES, C_LIST>(p, q, s_list + grp * C_LIST, active, slot, sub, lane, me, fx, fy,
-                                                   fz);
+    bal_pair_sum<MODE, BATCH, 2, H_LANES, C_LIST, BEAD_HET>(p, q, s_list + grp * C_LIST, active, slot, sub, lane, me,
+                                                             fx, fy, fz);
   else
-    bal_pair_sum<MODE, BATCH, 1, T_LANES, T_LIST, true>(p, q, s_list + grp * T_LIST, active, slot, sub, lane, me, fx,
-                                                        fy, fz);
+    bal_pair_sum<MODE, BATCH, 1, T_LANES, T_LIST, BEAD_SHORT>(p, q, s_list + grp * T_LIST, active, slot, sub, lane,
+                                                              me, fx, fy, fz);
   // The epilogue of both wavefronts' beads runs on the first one, so the
   // per-bead work is issued once per workgroup instead of once per
   // wavefront: the second hands over each bead's total, slot, position and

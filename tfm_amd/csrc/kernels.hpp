@@ -11,12 +11,12 @@ namespace ccpmi {
 // bead exist (calc_bf's lim_l/lim_u test, chrsim.cu:43-56).
 constexpr uint32_t ID_BITS = 26;
 constexpr uint32_t ID_MASK = (1u << ID_BITS) - 1u;
-constexpr uint32_t W_HET = 1u << 26;
+constexpr uint32_t W_B3 = 1u << 26; // r[i+2] - r[i+1]
 constexpr uint32_t W_LAD = 1u << 27;
 constexpr uint32_t W_B0 = 1u << 28; // r[i-1] - r[i-2]
 constexpr uint32_t W_B1 = 1u << 29; // r[i]   - r[i-1]
 constexpr uint32_t W_B2 = 1u << 30; // r[i+1] - r[i]
-constexpr uint32_t W_B3 = 1u << 31; // r[i+2] - r[i+1]
+constexpr uint32_t W_HET = 1u << 31; // the sign bit: a het partner is a negative word
 
 // cells per scan block of the order kernel; each block reads the totals of
 // the blocks before it, so fewer, larger blocks keep that prefix cheap
