@@ -479,6 +479,7 @@ __device__ __forceinline__ void pair_full(const KParams &p, float rx, float ry, 
 struct PairCtx {
   float rx, ry, rz, e_wca, e_het;
   uint32_t im1, het;
+  uint32_t nim1; // 1 - i
 };
 __device__ __forceinline__ PairCtx pair_ctx(const KParams &p, float4 me) {
   PairCtx c;
@@ -487,11 +488,21 @@ __device__ __forceinline__ PairCtx pair_ctx(const KParams &p, float4 me) {
   c.ry = me.y;
   c.rz = me.z;
   c.im1 = (w & ID_MASK) - 1u;
+  c.nim1 = 0u - c.im1;
   c.het = w & W_HET;
   c.e_wca = p.eps_wca;
   c.e_het = p.eps_het;
-  asm("" : "+v"(c.im1), "+v"(c.e_wca), "+v"(c.e_het));
+  asm("" : "+v"(c.im1), "+v"(c.nim1), "+v"(c.e_wca), "+v"(c.e_het));
   return c;
+}
+
+// (the partner's index) - (i - 1), the index taken from the low 24 bits of
+// the partner's word by the multiply-add itself
+__device__ __forceinline__ uint32_t index_gap(uint32_t wj, uint32_t nim1) {
+  static_assert(ID_BITS == 24, "v_mad_u32_u24 reads a 24-bit index");
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, 1, %2" : "=v"(r) : "v"(wj), "v"(nim1));
+  return r;
 }
 
 // Kind of bead of a pair term, when the caller knows it: any, a bead with
@@ -509,7 +520,7 @@ __device__ __forceinline__ void pair_term(const KParams &p, const PairCtx &c, fl
   const uint32_t wj = wbits(o);
   const float dx = c.rx - o.x, dy = c.ry - o.y, dz = c.rz - o.z;
   const float d2 = dx * dx + dy * dy + dz * dz;
-  const bool nb = (wj & ID_MASK) - c.im1 > 2u;
+  const bool nb = index_gap(wj, c.nim1) > 2u;
   const bool hhi = KIND == BEAD_SHORT ? false : KIND == BEAD_HET ? (int32_t)wj < 0 : (wj & c.het) != 0u;
   float fac;
   if constexpr (MODE == MODE_CCP) {

@@ -9,7 +9,7 @@ namespace ccpmi {
 // Bits of the 32-bit word stored in .w of every bead float4:
 // bead index, type flags, and which of ccp's four bond vectors around the
 // bead exist (calc_bf's lim_l/lim_u test, chrsim.cu:43-56).
-constexpr uint32_t ID_BITS = 26;
+constexpr uint32_t ID_BITS = 24; // the low 24 bits: v_mad_u32_u24 reads the index as it is
 constexpr uint32_t ID_MASK = (1u << ID_BITS) - 1u;
 constexpr uint32_t W_B3 = 1u << 26; // r[i+2] - r[i+1]
 constexpr uint32_t W_LAD = 1u << 27;
