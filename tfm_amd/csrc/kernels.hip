@@ -67,6 +67,13 @@ __device__ __forceinline__ float ccp_lbs(double lfac, float d, float lco) {
   const float d2 = nd * nd;
   return (float)(lfac * ((double)(d2 * d2 * d2) - 1.0));
 }
+// Squared distance in one fixed rounding order, fma(dz, dz, fma(dy, dy,
+// dx * dx)). Written as a sum of three products it leaves the compiler free
+// to fuse either product of the first sum, and two inlined copies of the
+// same pair term may be fused differently; a sum over a list would then
+// depend on which copy took each entry.
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
+
 // the element at byte offset `off` of `base` (a 32-bit offset: the load takes
 // the scalar base and the offset register as they are)
 template <class T> __device__ __forceinline__ T at_byte(const T *base, uint32_t off) {
@@ -147,6 +154,18 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 // sum over the 64 lanes (all active), the same in every lane
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), 63);
+}
+
+// maximum over the 64 lanes (all active), the same in every lane: the
+// steps of wave_incl_scan with max for +
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+  v = max(v, dpp_u<DPP_SHR1>(v));
+  v = max(v, dpp_u<DPP_SHR2>(v));
+  v = max(v, dpp_u<DPP_SHR4>(v));
+  v = max(v, dpp_u<DPP_SHR8>(v));
+  v = max(v, dpp_u<DPP_BCAST15, 0xF, 0xA>(v));
+  v = max(v, dpp_u<DPP_BCAST31, 0xF, 0xC>(v));
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // atomicAdd(base[key], 1) for every active lane, one atomic per distinct key
@@ -238,7 +257,7 @@ __device__ __forceinline__ void pair_short(const KParams &p, float rx, float ry,
   const uint32_t wj = wbits(o);
   const uint32_t i = wi & ID_MASK, j = wj & ID_MASK;
   const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
-  const float d2 = dx * dx + dy * dy + dz * dz;
+  const float d2 = dist2(dx, dy, dz);
   const bool nb = (j > i + 1u) || (i > j + 1u);
   float fac;
   if constexpr (MODE == MODE_CCP) {
@@ -269,7 +288,7 @@ __device__ __forceinline__ void pair_tail(const KParams &p, float rx, float ry, 
   const uint32_t wj = wbits(o);
   const uint32_t i = wi & ID_MASK, j = wj & ID_MASK;
   const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
-  const float d2 = dx * dx + dy * dy + dz * dz;
+  const float d2 = dist2(dx, dy, dz);
   const bool nb = (j > i + 1u) || (i > j + 1u);
   float fac;
   if constexpr (MODE == MODE_CCP) {
@@ -291,7 +310,7 @@ __device__ __forceinline__ void pair_tail(const KParams &p, float rx, float ry, 
 __device__ __forceinline__ void lbs_force(const KParams &p, float rx, float ry, float rz, float4 o, float &fx,
                                           float &fy, float &fz) {
   const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
-  const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+  const float d = sqrtf(dist2(dx, dy, dz));
   const float fac = d <= p.lco ? ccp_lbs(p.lfac_d, d, p.lco) : 0.0f;
   fx += fac * dx;
   fy += fac * dy;
@@ -442,7 +461,7 @@ __device__ __forceinline__ void pair_full(const KParams &p, float rx, float ry, 
   const uint32_t wj = wbits(o);
   const uint32_t i = wi & ID_MASK, j = wj & ID_MASK;
   const float dx = rx - o.x, dy = ry - o.y, dz = rz - o.z;
-  const float d2 = dx * dx + dy * dy + dz * dz;
+  const float d2 = dist2(dx, dy, dz);
   // |i - j| > 1 as one unsigned compare: j - (i - 1) wraps past 2 unless
   // j is i - 1, i or i + 1
   const bool nb = j - (i - 1u) > 2u;
@@ -519,7 +538,7 @@ __device__ __forceinline__ void pair_term(const KParams &p, const PairCtx &c, fl
   static_assert(W_HET == 0x80000000u, "the het flag is the sign bit");
   const uint32_t wj = wbits(o);
   const float dx = c.rx - o.x, dy = c.ry - o.y, dz = c.rz - o.z;
-  const float d2 = dx * dx + dy * dy + dz * dz;
+  const float d2 = dist2(dx, dy, dz);
   const bool nb = index_gap(wj, c.nim1) > 2u;
   const bool hhi = KIND == BEAD_SHORT ? false : KIND == BEAD_HET ? (int32_t)wj < 0 : (wj & c.het) != 0u;
   float fac;
@@ -1105,6 +1124,13 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   total += clen;
   uint32_t off = clen + incl - tot;
   const bool listed = total <= (uint32_t)LCAP;
+  // The gathers run in rounds common to the wavefront, up to its longest
+  // list: a wavefront issues every round that any of its groups needs, so
+  // the shorter lists are padded with the bead's own slot (a term of exactly
+  // zero: |i - j| <= 1 is excluded) instead of ending early. Each lane still
+  // adds its own entries in list order, so the sums are unchanged.
+  const uint32_t wmax = wave_max(listed ? total : 0u);
+  const uint32_t s4 = self << 4;
   // the list holds byte offsets (slot x 16) into the ordered positions, so a
   // gather needs no address arithmetic
   if (listed) {
@@ -1114,6 +1140,10 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
       uint32_t b = lo[m] << 4;
       for (uint32_t k = 0; k < len[m]; ++k, b += 16u) list[off++] = b;
     }
+    // pads up to the last entry a lane reads: wmax rounded up to G (LCAP is
+    // a multiple of G, so they stay in the list)
+    const uint32_t lim = (wmax + (uint32_t)G - 1u) / (uint32_t)G * (uint32_t)G;
+    for (uint32_t k = total + sub; k < lim; k += G) list[k] = s4;
   }
   // the list is read by the lanes of the same wavefront only: complete the
   // LDS writes and order them before the reads, without a block barrier
@@ -1123,15 +1153,12 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
 
   const PairCtx pc = pair_ctx(p, me);
   if (listed) {
-    // Rounds of gathers in flight per lane (entries n, n + 8, n + 16, ...):
-    // full ones of four while the bead's list holds 32 more, then one tail
-    // round, four wide if some bead of the wavefront has more than 24 left
-    // and three otherwise. The tail is padded with the bead's own slot, whose
-    // term is exactly zero (|i - j| <= 1 is excluded), so each lane adds the
-    // same entries in the same order whatever the width.
+    // Rounds of eight gathers in flight per lane (entries n, n + G, n + 2G,
+    // ...) while the wavefront's longest list holds 8G more, then one round
+    // of just the gathers the rest needs (wmax and base are the same in all
+    // lanes, so its width is a scalar branch per gather).
     uint32_t base = 0;
-    // long lists (het beads): eight gathers in flight
-    for (; base + 8 * G <= total; base += 8 * G) {
+    for (; base + 8 * G <= wmax; base += 8 * G) {
       const uint32_t n = base + sub;
       float4 e[8];
 #pragma unroll
@@ -1139,27 +1166,15 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
 #pragma unroll
       for (int k = 0; k < 8; ++k) pair_term<MODE, KIND>(p, pc, e[k], fx, fy, fz);
     }
-    for (; base + 4 * G <= total; base += 4 * G) {
-      const uint32_t n = base + sub;
-      const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, list[n + G]),
-                   c = at_byte(q.fspos, list[n + 2 * G]), d = at_byte(q.fspos, list[n + 3 * G]);
-      pair_term<MODE, KIND>(p, pc, a, fx, fy, fz);
-      pair_term<MODE, KIND>(p, pc, b, fx, fy, fz);
-      pair_term<MODE, KIND>(p, pc, c, fx, fy, fz);
-      pair_term<MODE, KIND>(p, pc, d, fx, fy, fz);
-    }
-    const uint32_t rem = total - base, n = base + sub, s4 = self << 4;
-    const bool wide = __ballot(rem > 3 * G) != 0ull;
-    if (n < total) {
-      const uint32_t jb = n + G < total ? list[n + G] : s4;
-      const uint32_t jc = n + 2 * G < total ? list[n + 2 * G] : s4;
-      const float4 a = at_byte(q.fspos, list[n]), b = at_byte(q.fspos, jb), c = at_byte(q.fspos, jc);
-      float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (wide) d = at_byte(q.fspos, n + 3 * G < total ? list[n + 3 * G] : s4);
-      pair_term<MODE, KIND>(p, pc, a, fx, fy, fz);
-      pair_term<MODE, KIND>(p, pc, b, fx, fy, fz);
-      pair_term<MODE, KIND>(p, pc, c, fx, fy, fz);
-      if (wide) pair_term<MODE, KIND>(p, pc, d, fx, fy, fz);
+    const uint32_t rem = wmax - base, n = base + sub;
+    if (rem > 0u) {
+      float4 e[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if ((uint32_t)(k * G) < rem) e[k] = at_byte(q.fspos, list[n + k * G]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if ((uint32_t)(k * G) < rem) pair_term<MODE, KIND>(p, pc, e[k], fx, fy, fz);
     }
   } else {
     // crowded neighbourhood: each lane scans its own rows, and every eighth
@@ -1742,7 +1757,7 @@ __global__ __launch_bounds__(BEAD_BLOCK) void k_overlaps(KParams p, const float4
         const float4 o = spos[k];
         const uint32_t j = wbits(o) & ID_MASK;
         const float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
-        if (j > i + 1u && dx * dx + dy * dy + dz * dz < dmin2) ++n;
+        if (j > i + 1u && dist2(dx, dy, dz) < dmin2) ++n;
       }
   }
   const uint32_t tot = wave_sum(n);
