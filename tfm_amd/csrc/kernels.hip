@@ -960,6 +960,7 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
                                              uint32_t self, int sub, int lane, float4 me, float &fx, float &fy,
                                              float &fz) {
   static_assert(G == 8 || (G == 2 && R == 1), "eight lanes per bead, or two at reach 1");
+  static_assert(LCAP % (8 * G) == 0, "the gather rounds end on the list's last entry");
   // eight lanes: the group's lanes hold the y and z gaps of offsets -R..R
   // and each row fetches its two; two lanes compute all three themselves
   constexpr bool LANE_GAPS = G >= 2 * R + 1;
@@ -1168,10 +1169,16 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
     }
     const uint32_t rem = wmax - base, n = base + sub;
     if (rem > 0u) {
+      // the eight list entries read together (all inside the group's LCAP
+      // entries: base is a multiple of 8G below LCAP, a multiple of 8G), the
+      // gathers only for those in use
+      uint32_t j[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) j[k] = list[n + k * G];
       float4 e[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if ((uint32_t)(k * G) < rem) e[k] = at_byte(q.fspos, list[n + k * G]);
+        if ((uint32_t)(k * G) < rem) e[k] = at_byte(q.fspos, j[k]);
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if ((uint32_t)(k * G) < rem) pair_term<MODE, KIND>(p, pc, e[k], fx, fy, fz);
