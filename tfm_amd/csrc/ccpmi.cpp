@@ -501,8 +501,9 @@ hipGraphExec_t ccp_sim::step_graph(uint32_t len, int mode) {
   return ge;
 }
 
-// n steps without contact sampling: whole graphs of graph_steps, then the
-// rest as graphs of the powers of two in its binary expansion.
+// n steps without contact sampling: the rest below graph_steps as graphs of
+// the powers of two in its binary expansion, then whole graphs of
+// graph_steps.
 void ccp_sim::run_chunk(uint64_t n, int mode) {
   if (n == 0) return;
   if (!use_graphs) {
@@ -513,12 +514,15 @@ void ccp_sim::run_chunk(uint64_t n, int mode) {
   const uint64_t G = graph_steps;
   const int cf = gf.clean, ch = gh.clean;
   const bool vh = gh.valid;
-  if (n >= G) {
+  // the blocks below G first, shortest first: the device starts on the first
+  // block while the host still submits the longer ones
+  const uint64_t full = n / G, rest = n % G;
+  for (uint32_t len = 1; len <= (uint32_t)rest; len <<= 1)
+    if (rest & len) hip_check(hipGraphLaunch(step_graph(len, mode), stream));
+  if (full) {
     hipGraphExec_t ge = step_graph((uint32_t)G, mode);
-    for (; n >= G; n -= G) hip_check(hipGraphLaunch(ge, stream));
+    for (uint64_t b = 0; b < full; ++b) hip_check(hipGraphLaunch(ge, stream));
   }
-  for (uint32_t len = 1u << 12; len > 0; len >>= 1)
-    if (n & len) hip_check(hipGraphLaunch(step_graph(len, mode), stream));
   gf.clean = cf;
   gh.clean = ch;
   gh.valid = mode != MODE_ICG ? vh : false;
@@ -782,6 +786,7 @@ void ccp_sim::generate_initial_condition() {
       run_steps(spf, MODE_ICG);
       if (!take_overflow()) break;
       hip_check(hipMemcpyAsync(r.p, aux.p, (size_t)N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+      terms_mode = -1;
       step -= spf;
       dstep_dirty = true;
       grow_capacity();

@@ -91,6 +91,9 @@ struct ccp_sim {
   uint32_t N = 0, n_l = 0, fpf = 0, spf = 0, tpb = 0;
   float T = 0, R_n = 0, R_o = 0, R_b = 0;
   float noc = 0, boc = 0, nod = 0, bod = 0, d_m = 0;
+  // pair set (MODE_*) whose bonded and wall terms fbond holds for the
+  // positions in r (-1: none, or stale after a host-side change of r)
+  int terms_mode = -1;
   float dt = 0, sd = 0;
   uint64_t seed = 0;
   uint32_t replica = 0; // > 0: same bead types as replica 0, its own walk and noise
@@ -276,7 +279,10 @@ struct ccp_sim {
   bool take_overflow();
   void grow_capacity();
   void ensure_grids();
-  void invalidate() { gf.valid = gh.valid = gx.valid = false; }
+  void invalidate() {
+    gf.valid = gh.valid = gx.valid = false;
+    terms_mode = -1;
+  }
   void sync_step();
   NoiseJob noise_job(uint32_t ahead) const {
     return NoiseJob{znoise.p, dstep.p, ahead, N, kp.seed_lo, kp.seed_hi, nullptr, nullptr, 0};
@@ -287,15 +293,18 @@ struct ccp_sim {
   BondJob bond_job(const float4 *pos, int mode) const {
     return BondJob{fbond.p, pos, N, mode, nullptr, nullptr, 0};
   }
-  // before the first stage of a run: the normals of the current step (unless
-  // held already) and the bonded terms of the current positions
+  // before the first stage of a run: the normals of the current step and the
+  // bonded terms of the current positions, unless held already (the order
+  // pass after each stage leaves both for the next one)
   void prime_terms(int mode) {
     sync_step();
     const NoiseJob nj = noise_job(0);
     const BondJob bj = bond_job(r.p, mode);
     const KParams k = mode_params(mode);
-    launch_bead_terms(stream, noise_step != step ? &nj : nullptr, k, bond_pass() ? &bj : nullptr);
+    const bool noise = noise_step != step, bond = bond_pass() && terms_mode != mode;
+    launch_bead_terms(stream, noise ? &nj : nullptr, k, bond ? &bj : nullptr);
     noise_step = step;
+    terms_mode = bond_pass() ? mode : -1;
   }
   // `mid` (optional) is recorded between the force and the order launches
   void launch_stage(int mode, int stage, hipEvent_t mid = nullptr);
