@@ -9,6 +9,7 @@
 #include "stats.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -275,8 +276,9 @@ void ccp_sim::fill_kparams() {
 void ccp_sim::allocate() {
   hip_check(hipGetDevice(&dev));
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-  hip_check(hipHostMalloc(reinterpret_cast<void **>(&hflags), FLAG_COUNT * sizeof(uint32_t), hipHostMallocDefault));
+  hip_check(hipHostMalloc(reinterpret_cast<void **>(&hsig), 64, hipHostMallocMapped | hipHostMallocCoherent));
+  std::memset(hsig, 0, 64);
+  hip_check(hipHostGetDevicePointer(reinterpret_cast<void **>(&hsig_dev), hsig, 0));
   r.alloc(N); er.alloc(N); pstar.alloc(N); aux.alloc(N); znoise.alloc(N);
   if (bond_pass()) fbond.alloc(N);
   if (lanes == 8 && integrate_pass) fpair.alloc(N);
@@ -352,12 +354,31 @@ void ccp_sim::download_positions() {
 // Waits for everything enqueued on the stream, then raises any error flag:
 // the flags are copied into pinned host memory behind the stream's work and
 // one event wait (a spin, not an interrupt) covers both.
+// Waits for the stream's work and checks the error flags: k_signal, queued
+// behind that work, writes the flags and then a new sequence number into
+// pinned host memory, and the host polls for the number (a few microseconds
+// fewer than a copy into host memory followed by an event wait). Every few
+// thousand polls the stream is queried, so that a failed launch is reported
+// instead of waited for.
 void ccp_sim::check_flags() {
-  hip_check(hipMemcpyAsync(hflags, flags.p, FLAG_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-  hip_check(hipEventRecord(done, stream));
-  hip_check(hipEventSynchronize(done));
+  const uint32_t seq = ++sig_seq;
+  launch_signal(stream, flags.p, hsig_dev, seq);
+  hip_check(hipGetLastError());
+  volatile uint32_t *sig = hsig;
+  for (uint32_t k = 1; sig[0] != seq; ++k) {
+    if ((k & 4095u) == 0u) {
+      const hipError_t e = hipStreamQuery(stream);
+      if (e == hipSuccess) {
+        std::atomic_thread_fence(std::memory_order_seq_cst);
+        if (sig[0] != seq) throw Error(CCP_ERR_HIP, "stream drained without its completion signal");
+        break;
+      }
+      if (e != hipErrorNotReady) hip_check(e);
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
   uint32_t f[FLAG_COUNT];
-  std::memcpy(f, hflags, sizeof(f));
+  for (int k = 0; k < FLAG_COUNT; ++k) f[k] = sig[1 + k];
   if (f[FLAG_OVERFLOW] || f[FLAG_OUTSIDE] || f[FLAG_NONFINITE]) {
     hip_check(hipMemset(flags.p, 0, sizeof(f)));
     if (f[FLAG_NONFINITE]) throw Error(CCP_ERR_STATE, "non-finite bead position");

@@ -1968,6 +1968,18 @@ void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t 
   k_order<<<blocks, SCAN_BLOCK, 0, s>>>(a, jb, flags, step_inc, nj, p, bj);
 }
 
+__global__ __launch_bounds__(64) void k_signal(const uint32_t *flags, uint32_t *host, uint32_t seq) {
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < FLAG_COUNT; ++k) host[1 + k] = flags[k];
+  // the flags reach host memory before the sequence number does
+  __threadfence_system();
+  host[0] = seq;
+}
+
+void launch_signal(hipStream_t s, const uint32_t *flags, uint32_t *host, uint32_t seq) {
+  k_signal<<<1, 64, 0, s>>>(flags, host, seq);
+}
+
 void launch_bead_terms(hipStream_t s, const NoiseJob *noise, const KParams &p, const BondJob *bond) {
   NoiseJob nj{};
   BondJob bj{};

@@ -197,6 +197,10 @@ void launch_order(hipStream_t s, const OrderJob &a, const OrderJob *b, uint32_t 
                   const NoiseJob *noise = nullptr, const KParams *bp = nullptr, const BondJob *bond = nullptr);
 // the same per-bead terms on their own (before the first stage of a run)
 void launch_bead_terms(hipStream_t s, const NoiseJob *noise, const KParams &p, const BondJob *bond);
+// Copy the FLAG_COUNT error flags into host[1..] and then seq into host[0]
+// (host: device address of coherent pinned host memory), behind every
+// launch ahead of it in the stream.
+void launch_signal(hipStream_t s, const uint32_t *flags, uint32_t *host, uint32_t seq);
 void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q);
 // Contacts of the beads in slots [slot0, slot0 + p.N) (bead indices from
 // id0) on p.gf, binned by (index - id0) / contact_bin_size.

@@ -139,10 +139,10 @@ struct ccp_sim {
   hipStream_t stream = nullptr;
   DevBuf<float4> r, er, pstar, lr, lspos, aux, fpair, znoise, fbond;
   DevBuf<uint32_t> lstart, flags, cmap;
-  // pinned host copy of the error flags, and the event a synchronization
-  // waits on (created without hipEventBlockingSync: the wait spins)
-  uint32_t *hflags = nullptr;
-  hipEvent_t done = nullptr;
+  // completion word and flags written by k_signal: coherent pinned host
+  // memory (host and device addresses) and the last sequence number sent
+  uint32_t *hsig = nullptr, *hsig_dev = nullptr;
+  uint32_t sig_seq = 0;
   DevBuf<uint64_t> dstep;
   DevBuf<unsigned long long> dcount;
   // one uniform grid: binning buffers + ordered cell list
@@ -233,8 +233,7 @@ struct ccp_sim {
     gx.free_();
     dstep.free_();
     dcount.free_();
-    if (hflags) (void)hipHostFree(hflags);
-    if (done) (void)hipEventDestroy(done);
+    if (hsig) (void)hipHostFree(hsig);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
