@@ -351,9 +351,6 @@ void ccp_sim::download_positions() {
   }
 }
 
-// Waits for everything enqueued on the stream, then raises any error flag:
-// the flags are copied into pinned host memory behind the stream's work and
-// one event wait (a spin, not an interrupt) covers both.
 // Waits for the stream's work and checks the error flags: k_signal, queued
 // behind that work, writes the flags and then a new sequence number into
 // pinned host memory, and the host polls for the number (a few microseconds

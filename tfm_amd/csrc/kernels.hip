@@ -1129,7 +1129,8 @@ __device__ __forceinline__ void bal_pair_sum(const KParams &p, const KPtrs &q, u
   // list: a wavefront issues every round that any of its groups needs, so
   // the shorter lists are padded with the bead's own slot (a term of exactly
   // zero: |i - j| <= 1 is excluded) instead of ending early. Each lane still
-  // adds its own entries in list order, so the sums are unchanged.
+  // adds its own entries in list order, so a bead's sum does not depend on
+  // the other beads of its wavefront.
   const uint32_t wmax = wave_max(listed ? total : 0u);
   const uint32_t s4 = self << 4;
   // the list holds byte offsets (slot x 16) into the ordered positions, so a
