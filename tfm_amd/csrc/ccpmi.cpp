@@ -481,17 +481,21 @@ void ccp_sim::launch_stage(int mode, int stage, hipEvent_t mid) {
   q.fpair = fpair.p;
   q.noise = znoise.p;
   q.fbond = bond_pass() ? fbond.p : nullptr;
-  launch_force(stream, mode, stage, k, q);
+  // The next step's normals (the step counter still holds this step until
+  // the order pass after stage 2 advances it): drawn in the tail of the
+  // second stage's force launch by the type-ordered kernel, which does not
+  // read them, otherwise by the order pass after the first stage.
+  const NoiseJob nj = noise_job(1);
+  const bool drew = launch_force(stream, mode, stage, k, q, typed && stage == 2 ? &nj : nullptr);
+  if (typed && stage == 2 && !drew) throw Error(CCP_ERR_STATE, "type-ordered force launch without its normals");
   if (mid) hip_check(hipEventRecord(mid, stream));
   const OrderJob jf = gf.order_job(gf.g, typed, (uint32_t)kp.n_het, N, gf.spos.p, gf.cstart.p, nullptr);
   OrderJob jh{gh.g, gh.count.p, gh.bs(gh.clean), gh.bs(gh.clean ^ 1), gh.bucket.p, gh.spos.p, nullptr,
               gh.cstart.p};
-  // With the order pass: after the first stage the counter still holds this
-  // step, so draw the next one's normals; the bonded terms of q.out0.
-  const NoiseJob nj = noise_job(1);
+  // and with the order pass, the bonded terms of q.out0
   const BondJob bj = bond_job(q.out0, mode);
-  launch_order(stream, jf, het ? &jh : nullptr, flags.p, stage == 2 ? dstep.p : nullptr, stage == 1 ? &nj : nullptr,
-               &k, bond_pass() ? &bj : nullptr);
+  launch_order(stream, jf, het ? &jh : nullptr, flags.p, stage == 2 ? dstep.p : nullptr,
+               stage == 1 && !typed ? &nj : nullptr, &k, bond_pass() ? &bj : nullptr);
   gf.clean ^= 1;
   if (het) gh.clean ^= 1;
   else gh.valid = false;

@@ -1350,10 +1350,19 @@ static_assert(BAL_BLOCK == 128, "the epilogue hand-over pairs the workgroup's tw
 static_assert(8 * C_LIST * 4 >= 8 * 3 * 16 && 32 * T_LIST * 4 >= 32 * 3 * 16,
               "a wavefront's list memory holds the hand-over records of its beads");
 template <int MODE, int STAGE, bool BATCH>
-__global__ __launch_bounds__(BAL_BLOCK) void k_force_typed(KParams p, KPtrs q) {
+__global__ __launch_bounds__(BAL_BLOCK) void k_force_typed(KParams p, KPtrs q, NoiseJob nj) {
   __shared__ __align__(16) uint32_t s_list[T_LDS];
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const int b = (int)blockIdx.x;
+  const int t_blocks = (p.N - p.n_het + T_BEADS_PER_BLOCK - 1) / T_BEADS_PER_BLOCK;
+  // Workgroups past the force ones (when the launch has them) draw the next
+  // step's normals, one lane per bead: dispatched last, they run in the
+  // launch's tail, on SIMDs that the force wavefronts leave idle.
+  if (b >= p.het_blocks + t_blocks) {
+    const uint32_t g = (uint32_t)(b - p.het_blocks - t_blocks) * BAL_BLOCK + threadIdx.x;
+    if (nj.z && g < nj.n) fill_noise(nj, g);
+    return;
+  }
   const bool het_part = b < p.het_blocks;
   int t, sub, grp;
   uint32_t end;
@@ -1365,7 +1374,7 @@ __global__ __launch_bounds__(BAL_BLOCK) void k_force_typed(KParams p, KPtrs q) {
   } else {
     sub = lane % T_LANES;
     grp = wave * (64 / T_LANES) + lane / T_LANES;
-    t = p.n_het + xcd_remap(b - p.het_blocks, (int)gridDim.x - p.het_blocks) * T_BEADS_PER_BLOCK + grp;
+    t = p.n_het + xcd_remap(b - p.het_blocks, t_blocks) * T_BEADS_PER_BLOCK + grp;
     end = (uint32_t)p.N;
   }
   const bool in = (uint32_t)t < end;
@@ -2000,15 +2009,20 @@ static void force_variant(hipStream_t s, int stage, const KParams &p, const KPtr
 }
 
 template <int MODE, bool BATCH, int R>
-static void force_balanced(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
+static bool force_balanced(hipStream_t s, int stage, const KParams &p, const KPtrs &q, const NoiseJob *nz) {
   if constexpr (R == 2) {
     if (p.typed && q.work) {
-      // het beads in het_blocks workgroups of 16, the others in workgroups of 64
-      const unsigned gt = (unsigned)p.het_blocks + blocks_for(p.N - p.n_het, T_BEADS_PER_BLOCK);
-      if (stage == 0) k_force_typed<MODE, 0, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q);
-      else if (stage == 1) k_force_typed<MODE, 1, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q);
-      else k_force_typed<MODE, 2, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q);
-      return;
+      // het beads in het_blocks workgroups of 16, the others in workgroups of
+      // 64, then (stage 2, when asked) the next step's normals, 128 beads per
+      // workgroup
+      NoiseJob z{};
+      if (stage == 2 && nz && nz->z) z = *nz;
+      const unsigned gt = (unsigned)p.het_blocks + blocks_for(p.N - p.n_het, T_BEADS_PER_BLOCK) +
+                          (z.z ? blocks_for((int)z.n, BAL_BLOCK) : 0u);
+      if (stage == 0) k_force_typed<MODE, 0, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q, z);
+      else if (stage == 1) k_force_typed<MODE, 1, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q, z);
+      else k_force_typed<MODE, 2, BATCH><<<gt, BAL_BLOCK, 0, s>>>(p, q, z);
+      return z.z != nullptr;
     }
   }
   const unsigned g = blocks_for(p.N, C_BEADS_PER_BLOCK);
@@ -2019,41 +2033,42 @@ static void force_balanced(hipStream_t s, int stage, const KParams &p, const KPt
     if (stage == 0) k_integrate<MODE, 0, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
     else if (stage == 1) k_integrate<MODE, 1, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
     else k_integrate<MODE, 2, BATCH><<<gi, BEAD_BLOCK, 0, s>>>(p, q);
-    return;
+    return false;
   }
   if (p.rounds == 4) {
     const unsigned g4 = blocks_for(p.N, C_BEADS_PER_BLOCK * 4);
     if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 4><<<g4, BAL_BLOCK, 0, s>>>(p, q);
     else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 4><<<g4, BAL_BLOCK, 0, s>>>(p, q);
     else k_force_bal<MODE, 2, BATCH, R, true, 4><<<g4, BAL_BLOCK, 0, s>>>(p, q);
-    return;
+    return false;
   }
   if (p.rounds == 2) {
     const unsigned g2 = blocks_for(p.N, C_BEADS_PER_BLOCK * 2);
     if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 2><<<g2, BAL_BLOCK, 0, s>>>(p, q);
     else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 2><<<g2, BAL_BLOCK, 0, s>>>(p, q);
     else k_force_bal<MODE, 2, BATCH, R, true, 2><<<g2, BAL_BLOCK, 0, s>>>(p, q);
-    return;
+    return false;
   }
   if (p.rounds == 8) {
     const unsigned g8 = blocks_for(p.N, C_BEADS_PER_BLOCK * 8);
     if (stage == 0) k_force_bal<MODE, 0, BATCH, R, true, 8><<<g8, BAL_BLOCK, 0, s>>>(p, q);
     else if (stage == 1) k_force_bal<MODE, 1, BATCH, R, true, 8><<<g8, BAL_BLOCK, 0, s>>>(p, q);
     else k_force_bal<MODE, 2, BATCH, R, true, 8><<<g8, BAL_BLOCK, 0, s>>>(p, q);
-    return;
+    return false;
   }
   if (stage == 0) k_force_bal<MODE, 0, BATCH, R><<<g, BAL_BLOCK, 0, s>>>(p, q);
   else if (stage == 1) k_force_bal<MODE, 1, BATCH, R><<<g, BAL_BLOCK, 0, s>>>(p, q);
   else k_force_bal<MODE, 2, BATCH, R><<<g, BAL_BLOCK, 0, s>>>(p, q);
+  return false;
 }
 
 // kernel shape: cell reach 1 or 2; one lane per bead, one per plane, or
 // eight sharing a compacted candidate list
-template <int MODE, bool BATCH> static void force_shape(hipStream_t s, int stage, const KParams &p, const KPtrs &q) {
+template <int MODE, bool BATCH>
+static bool force_shape(hipStream_t s, int stage, const KParams &p, const KPtrs &q, const NoiseJob *nz) {
   if (p.lanes == C_LANES) {
-    if (p.reach == 2) force_balanced<MODE, BATCH, 2>(s, stage, p, q);
-    else force_balanced<MODE, BATCH, 1>(s, stage, p, q);
-    return;
+    if (p.reach == 2) return force_balanced<MODE, BATCH, 2>(s, stage, p, q, nz);
+    return force_balanced<MODE, BATCH, 1>(s, stage, p, q, nz);
   }
   if (p.reach == 2) {
     if (p.lanes == 1) force_variant<MODE, false, BATCH, 2, 1>(s, stage, p, q);
@@ -2062,26 +2077,26 @@ template <int MODE, bool BATCH> static void force_shape(hipStream_t s, int stage
     if (p.lanes == 1) force_variant<MODE, false, BATCH, 1, 1>(s, stage, p, q);
     else force_variant<MODE, false, BATCH, 1, 3>(s, stage, p, q);
   }
+  return false;
 }
 
-void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q) {
-  if (p.N <= 0) return;
+bool launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q, const NoiseJob *next_noise) {
+  if (p.N <= 0) return false;
   if (q.sys) {
     // batches: the WCA/LJ set (or the relaxation of the initial conditions)
     // on one all-bead grid per system
-    if (mode == MODE_ICG) force_shape<MODE_ICG, true>(s, stage, p, q);
-    else force_shape<MODE_LJ, true>(s, stage, p, q);
-    return;
+    if (mode == MODE_ICG) return force_shape<MODE_ICG, true>(s, stage, p, q, next_noise);
+    return force_shape<MODE_LJ, true>(s, stage, p, q, next_noise);
   }
   if (p.split && mode != MODE_ICG) {
     // short pairs on the fine grid, het-het tail on the het grid (reach 1, a lane per plane)
     if (mode == MODE_CCP) force_variant<MODE_CCP, true, false, 1, 3>(s, stage, p, q);
     else force_variant<MODE_LJ, true, false, 1, 3>(s, stage, p, q);
-    return;
+    return false;
   }
-  if (mode == MODE_ICG) force_shape<MODE_ICG, false>(s, stage, p, q);
-  else if (mode == MODE_CCP) force_shape<MODE_CCP, false>(s, stage, p, q);
-  else force_shape<MODE_LJ, false>(s, stage, p, q);
+  if (mode == MODE_ICG) return force_shape<MODE_ICG, false>(s, stage, p, q, next_noise);
+  if (mode == MODE_CCP) return force_shape<MODE_CCP, false>(s, stage, p, q, next_noise);
+  return force_shape<MODE_LJ, false>(s, stage, p, q, next_noise);
 }
 
 void launch_contacts_batch(hipStream_t s, const SysParams *sys, const uint32_t *sys_slot, int n_sys, int n, float rc,

@@ -201,7 +201,11 @@ void launch_bead_terms(hipStream_t s, const NoiseJob *noise, const KParams &p, c
 // (host: device address of coherent pinned host memory), behind every
 // launch ahead of it in the stream.
 void launch_signal(hipStream_t s, const uint32_t *flags, uint32_t *host, uint32_t seq);
-void launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q);
+// next_noise (optional, stage 2): normals for the type-ordered kernel to draw
+// in workgroups after its own, which run in the launch's tail; returns whether
+// it took them (false: the caller draws them elsewhere).
+bool launch_force(hipStream_t s, int mode, int stage, const KParams &p, const KPtrs &q,
+                  const NoiseJob *next_noise = nullptr);
 // Contacts of the beads in slots [slot0, slot0 + p.N) (bead indices from
 // id0) on p.gf, binned by (index - id0) / contact_bin_size.
 void launch_contacts(hipStream_t s, const KParams &p, const float4 *spos, const uint32_t *cstart, uint32_t *map,
