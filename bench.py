@@ -69,10 +69,13 @@ PARAMS = {
 #   force stage 2: R sorted pos 16 + R bonded/wall force 16 + R r+f dt/2+noise 16
 #                  + W r 16 + W bucket entry 16 + count atomic 4 = 84
 #   order:         R bucket entry 16 + W sorted pos 16 + R bead-ordered pos 16
-#                  + W bonded/wall force 16 = 64 per bead (+ W next normals 16
-#                  after stage 1), + R count 4 + W count 4 + W start 4 per cell
+#                  + W bonded/wall force 16 = 64 per bead, + R count 4
+#                  + W count 4 + W start 4 per cell
+#   the next step's normals, W 16 per bead once a step: by the second stage's
+#   force launch with the type-ordered work list, else by the first order pass
 BYTES_FORCE_PER_BEAD = {1: 100, 2: 84}
-BYTES_ORDER_PER_BEAD = {1: 80, 2: 64}
+BYTES_ORDER_PER_BEAD = {1: 64, 2: 64}
+BYTES_NORMALS_PER_BEAD = 16
 BYTES_ORDER_PER_CELL = 12
 # state the timed steps start from: the initial condition advanced by this
 # many untimed steps (the relaxed walk's first steps are slower than the
@@ -272,11 +275,14 @@ def main():
     typed = eight and reach2 and str(params.get("type_order", 1)) != "0"
     kname = "k_force_typed" if typed else "k_force_bal" if eight else "k_force"
     dominant = f"{kname}<LJ,{stage}>"
-    bytes_force = BYTES_FORCE_PER_BEAD[stage] * N
+    normals_force = typed and stage == 2
+    normals_order = not typed and stage == 1
+    bytes_force = (BYTES_FORCE_PER_BEAD[stage] + (BYTES_NORMALS_PER_BEAD if normals_force else 0)) * N
     achieved = bytes_force / (force_ms * 1e-3) / 1e9
     traffic = read_traffic(args.traffic)
     valu = read_valu(args.sq)
-    order_bytes = BYTES_ORDER_PER_BEAD[stage] * N + BYTES_ORDER_PER_CELL * sim.info.n_c
+    order_bytes = ((BYTES_ORDER_PER_BEAD[stage] + (BYTES_NORMALS_PER_BEAD if normals_order else 0)) * N
+                   + BYTES_ORDER_PER_CELL * sim.info.n_c)
     sim.synchronize()
 
     cpu = None if args.no_cpu else cpu_baseline(sim, args.cpu_seconds)
