@@ -151,6 +151,52 @@ def test_checkpoint_resume_is_bit_exact(tmp_path):
         s.close()
 
 
+def test_split_runs_and_host_changes_are_bit_exact():
+    # run_steps launches no bead-terms pass when the last order pass left the
+    # bonded terms and the normals of the current state (the normals drawn in
+    # the tail of the second force launch); a host-side change of the
+    # positions must discard them. Runs split into pieces, with a force
+    # evaluation and synchronizations between them, must match one run, and a
+    # handle whose positions were replaced must match a fresh one.
+    step("split runs and host changes, LJ set, type-ordered kernel")
+    shape = dict(force_field="lj", number_of_lbs=0, lanes_per_bead=8, cell_reach=2, cell_size=1.25)
+    a = small_sim(**shape)
+    a.generate_initial_condition()
+    a.set_time(0, 0.0, 0)
+    xyz, fl = a.positions, a.flags
+    b = small_sim(**shape)
+    b.flags = fl
+    b.positions = xyz
+    b.set_time(0, 0.0, 0)
+    a.run_steps(200)
+    a.synchronize()
+    b.run_steps(37)
+    b.synchronize()
+    b.compute_forces()
+    b.run_steps(100)
+    b.synchronize()
+    b.run_steps(63)
+    b.synchronize()
+    assert np.array_equal(a.positions, b.positions)
+    # new positions on a handle that has run, against a fresh handle
+    P = a.positions.copy()
+    P[::7] += np.float32(0.01)
+    T = a.time
+    b.positions = P
+    b.set_time(*T)
+    c = small_sim(**shape)
+    c.flags = fl
+    c.positions = P
+    c.set_time(*T)
+    for s in (b, c):
+        s.run_steps(100)
+        s.synchronize()
+    assert np.array_equal(b.positions, c.positions)
+    assert not np.array_equal(b.positions, a.positions)
+    for s in (a, b, c):
+        s.close()
+
+
 def test_gro_and_trr_formats(tmp_path):
     step("gro/trr formats")
     a = small_sim()
